@@ -1,0 +1,152 @@
+/*
+ * smac_amd.h -- C ABI of the MI355X (gfx950) implementation of miniSMAC2D's implicit
+ * pseudo-time iteration (one call of the reference `step`, ref/step.f:53-609).
+ *
+ * Conventions shared by every entry point
+ *  - Arrays are the reference's: Fortran order, j fastest.  A 3-vector field q(j,k,n) is
+ *    3 planes of jmax*kmax doubles; a 3x3 block field b(j,k,m,n) is 9 planes ordered
+ *    (m,n) = (1,1),(2,1),(3,1),(1,2),...,(3,3) (plane index (m-1) + 3*(n-1)), m = equation
+ *    (row), n = variable (column); rtxy(j,k,a,b) is 6 planes (a-1) + 2*(b-1).
+ *  - Indices handed to or returned from the library (BC regions, jres/kres) are 1-based.
+ *  - Every call returns an smac_status; on failure smac_last_error() names the cause.
+ *  - A handle is used from one host thread at a time; handles are independent (no global
+ *    mutable state besides the last-error text of each thread).
+ */
+#ifndef SMAC_AMD_H
+#define SMAC_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum smac_status {
+    SMAC_OK = 0,
+    SMAC_ERR_ARG = 1,          /* null pointer, bad size, inconsistent arrays            */
+    SMAC_ERR_IMPSCH = 2,       /* impsch not 1 (line relaxation) or 2 (LU-SGS)           */
+    SMAC_ERR_IFLXO = 3,        /* iflxo not 3 (only the third-order flux is reachable)   */
+    SMAC_ERR_NEQS = 4,         /* neqs not 1 or 3                                         */
+    SMAC_ERR_SIZE = 5,         /* jmax/kmax beyond the supported range                    */
+    SMAC_ERR_BC = 6,           /* unsupported or malformed boundary region                */
+    SMAC_ERR_NONFINITE = 7,    /* residual or update produced Inf/NaN                     */
+    SMAC_ERR_HIP = 8,          /* a HIP runtime call failed                               */
+    SMAC_ERR_NOGPU = 9         /* no gfx950 device visible                                */
+} smac_status;
+
+/* Solver settings: the reference's namelist $input (ref/initia.f:85-89) plus the values its
+ * inizone/initia hard-code (ref/initia.f:104-112, 418-437; ref/geom.f:72-76). */
+typedef struct smac_params {
+    double alpha;      /* angle of attack, degrees                                  */
+    double beta;       /* artificial compressibility                                */
+    double reynum;     /* Reynolds number; vnu = 1/reynum (ref/initia.f:324)        */
+    double dtau;       /* pseudo-time step                                          */
+    double dt;         /* SA time step (1e12 in the shipped input)                  */
+    double underr;     /* line-relaxation under-relaxation, 0 < underr <= 1          */
+    double cdis;       /* flux dissipation factor (1)                               */
+    double pin, pout;  /* inflow / outflow pressure (1.5, 1.0)                      */
+    double chord, xmom, ymom;   /* force normalisation and moment centre (1, .25, 0)  */
+    int impsch;        /* 1 = line relaxation (lrelax), 2 = LU-SGS (lusgs)           */
+    int ivis;          /* 1 = viscous                                                */
+    int iturb;         /* 1 = Spalart-Allmaras (needs ivis = 1)                      */
+    int iflxo;         /* flux order, must be 3                                      */
+    int njsp, nksp;    /* xi / eta line sweeps per iteration (2, 2)                  */
+    int nbcimp;        /* wake-injection frequency in sweeps (1)                     */
+    int ntjsp, ntksp;  /* SA j / k line sweeps (2, 0)                                */
+    int precision;     /* SMAC_FP64 or SMAC_FP32_LHS                                  */
+    int line_order;    /* SMAC_ORDER_REFERENCE or SMAC_ORDER_ZEBRA                    */
+} smac_params;
+
+enum { SMAC_FP64 = 0, SMAC_FP32_LHS = 1 };
+/* SMAC_ORDER_REFERENCE: lines in the reference's Gauss-Seidel order (per-iteration parity).
+ * SMAC_ORDER_ZEBRA: odd lines, then even lines, each colour fully parallel (a different
+ * iteration with the same fixed point; judged at convergence). */
+enum { SMAC_ORDER_REFERENCE = 0, SMAC_ORDER_ZEBRA = 1 };
+
+/* One BC region of bcmain.dat with indices already resolved to 1..jmax / 1..kmax
+ * (ref/rbc.f:384-396).  ibcval: 0 no-slip wall, 25 C-grid far field with point vortex,
+ * 31 outflow (fixed p), 60 wake cut; a region is a j-line (jbeg == jend) or a k-line. */
+typedef struct smac_bc_region {
+    int ibcval, jbeg, jend, kbeg, kend;
+} smac_bc_region;
+
+/* Per-iteration results: what the reference's solve/iterout keep in COMMON
+ * (/rmssr/, /rmssi/, /airfr/clift; ref/step.f:613-756, ref/smac2d.f:1709-1825). */
+typedef struct smac_iter_stats {
+    int nt;            /* iteration counter after this call                         */
+    int jres, kres;    /* location of resmax                                        */
+    int nonfinite;     /* 1 if the residual or q went non-finite                    */
+    double resmax, resmax0, divmax, turres;
+    double rhsnorm;    /* ||s||_2 entering the solver (ref/step.f:513-519)           */
+    double cl, cd, cm, clp, cdp;
+    double fxp, fyp, fxs, fys, fmom;
+} smac_iter_stats;
+
+typedef struct smac_handle smac_handle;
+
+/* ---- Tier 2: device-resident iteration ------------------------------------------------- */
+
+/* Set up one case on the current HIP device: grid, metrics (as ref/metric.f forms them),
+ * BC regions (wall list and wake pairs are derived as ref/rbc.f:972-990, 1160-1470 do),
+ * and the free-stream initial condition of ref/geom.f:88-131.
+ * Replaces the state solve() builds before its time loop (ref/smac2d.f:1239-1332). */
+smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *x,
+                        const double *y, const double *rtxy, const double *dj, int nreg,
+                        const smac_bc_region *regs, smac_handle **out);
+smac_status smac_destroy(smac_handle *h);
+
+/* Overwrite q (3 planes) and anut (1 plane); either may be NULL to keep it.  nt, resmax0
+ * and clift are set as given (a restart from a saved state). */
+smac_status smac_set_state(smac_handle *h, const double *q, const double *anut, int nt,
+                           double resmax0, double clift);
+/* Copy q (3 planes), anut, vnut (1 plane each) to the host; any pointer may be NULL. */
+smac_status smac_get_state(smac_handle *h, double *q, double *anut, double *vnut);
+
+/* One pass of the reference's time loop body (ref/smac2d.f:1386-1407): nt += 1, step
+ * (SA, assembly, BC rows, residual, line relaxation or LU-SGS, update), then the wall
+ * force integration that sets clift for the next pass.  Replaces step() at
+ * ref/smac2d.f:1395 together with the force/clift part of iterout at ref/smac2d.f:1407. */
+smac_status smac_step(smac_handle *h, smac_iter_stats *st);
+
+/* niter passes of smac_step without host round trips in between; stops early once
+ * rhsnorm <= drop * (rhsnorm of the first pass) when drop > 0.  *done = passes made. */
+smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *last, int *done);
+
+/* Copy a named device array to the host (diagnostics and kernel-level tests):
+ * "q","dq","s","anut","vnut" and the assembled LHS "btc","bjm","bjp","bkm","bkp" (FP64
+ * planes, as assembled before the solver runs). */
+smac_status smac_get_array(smac_handle *h, const char *name, double *out);
+
+/* Averaged device time (ms) of the last smac_run's passes, per kernel family:
+ * out[0] assembly+BC, [1] residual norms, [2] line factor, [3] line sweeps or LU-SGS,
+ * [4] SA, [5] update+forces; out[6] = whole pass. */
+smac_status smac_timings(smac_handle *h, double out[7]);
+
+/* ---- Tier 1: drop-ins at the reference's own Fortran call sites (host arrays) --------- */
+
+/* lrelax(jmax,kmax,q,dq,s,btc,bjm,bjp,bkm,bkp), ref/lrelax.f:51-221, called at
+ * ref/step.f:538.  Writes dq; writes s at the wake-cut points as bcimpds does
+ * (ref/bc.f:571-591).  btc/bjp/bkp are read only (the reference factors them in place and
+ * recomposes them; its caller never reads them afterwards).  wake holds npair (jt, jb)
+ * pairs: dq(jt,1) is injected from the average of points jt and jb at k=2. */
+smac_status smac_lrelax(const smac_params *p, int jmax, int kmax, int npair, const int *wake,
+                        const double *q, double *dq, double *s, const double *btc,
+                        const double *bjm, const double *bjp, const double *bkm,
+                        const double *bkp);
+
+/* lusgs(jmax,kmax,neqs,q,dq,rhs,btc,bjm,bjp,bkm,bkp,scr1), ref/lusgs.f:51-181, called at
+ * ref/step.f:552 (neqs=3) and ref/tursa.f:252 (neqs=1).  Writes dq, scr1 (the LU of btc)
+ * and rhs (destroyed as in the reference: the forward-sweep result, with the wake-cut
+ * points overwritten). */
+smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int npair,
+                       const int *wake, const double *q, double *dq, double *rhs,
+                       const double *btc, const double *bjm, const double *bjp,
+                       const double *bkm, const double *bkp, double *scr1);
+
+/* Text of the last error on this thread ("" if none). */
+const char *smac_last_error(void);
+/* Library version string. */
+const char *smac_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
