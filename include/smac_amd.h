@@ -111,14 +111,20 @@ smac_status smac_step(smac_handle *h, smac_iter_stats *st);
 smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *last, int *done);
 
 /* Copy a named device array to the host (diagnostics and kernel-level tests):
- * "q","dq","s","anut","vnut" and the assembled LHS "btc","bjm","bjp","bkm","bkp" (FP64
- * planes, as assembled before the solver runs). */
+ * "q","dq","s","anut","vnut","smin","sf","stc","ds","vort" and, with option "keep_lhs", the
+ * assembled LHS "btc","bjm","bjp","bkm","bkp" (FP64 planes as assembled, before the solver
+ * factors them). */
 smac_status smac_get_array(smac_handle *h, const char *name, double *out);
 
-/* Averaged device time (ms) of the last smac_run's passes, per kernel family:
- * out[0] assembly+BC, [1] residual norms, [2] line factor, [3] line sweeps or LU-SGS,
- * [4] SA, [5] update+forces; out[6] = whole pass. */
+/* Averaged device time (ms) per pass of the last smac_run (option "timing" on), per kernel
+ * family: out[0] SA, [1] assembly + BC rows, [2] residual norms, [3] line relaxation or
+ * LU-SGS, [4] update + wall forces, [5] unused (0), [6] whole pass. */
 smac_status smac_timings(smac_handle *h, double out[7]);
+
+/* Options: "keep_lhs" (1: keep a copy of the five assembled blocks of each pass for
+ * smac_get_array), "timing" (1: time kernel families with HIP events on the handle's
+ * stream).  No counterpart in the reference (diagnostics). */
+smac_status smac_set_option(smac_handle *h, const char *name, int value);
 
 /* ---- Tier 1: drop-ins at the reference's own Fortran call sites (host arrays) --------- */
 

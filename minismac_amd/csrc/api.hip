@@ -1,0 +1,597 @@
+// C ABI of the gfx950 implicit iteration (include/smac_amd.h): per-case device state, the
+// launch sequence of one pass of the reference's time loop, and the host-array drop-ins for
+// the reference's lrelax / lusgs call sites.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/smac_amd.h"
+#include "smac_kernels.h"
+
+using namespace smac;
+
+static thread_local std::string g_err;
+
+static smac_status fail(smac_status st, const std::string &msg)
+{
+    g_err = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(SMAC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+struct smac_handle {
+    smac_params p;
+    DevParams dp;
+    Dims d;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // grid (device)
+    double *x = nullptr, *y = nullptr, *met = nullptr, *dj = nullptr;
+    // flow state and LHS (device, FP64)
+    double *q = nullptr, *dq = nullptr, *s = nullptr;
+    double *blk[6] = {};   // btc bjm bjp bkm bkp scr1
+    double *lhs_copy[5] = {};
+    SaArrays sa = {};
+    DevStats *st = nullptr;
+    Scratch w = {};
+    int *pairs = nullptr;
+    WakePairs wake = {};
+    Walls walls = {};
+    std::vector<BcRegion> regs;
+    int nt = 0;
+    bool first_sa = true;
+    bool keep_lhs = false;
+    bool timing = false;
+    hipEvent_t ev[7] = {};
+    double t_acc[7] = {};
+    int t_n = 0;
+    std::vector<void *> allocs;
+};
+
+static bool check_gpu(std::string &why)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        why = "no HIP device visible";
+        return false;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) != hipSuccess) {
+        why = "hipGetDeviceProperties failed";
+        return false;
+    }
+    if (std::strncmp(pr.gcnArchName, "gfx950", 6) != 0) {
+        why = std::string("this library is built for gfx950; device is ") + pr.gcnArchName;
+        return false;
+    }
+    return true;
+}
+
+template <typename T>
+static smac_status dalloc(smac_handle *h, T **ptr, size_t n)
+{
+    void *p = nullptr;
+    HIP_TRY(hipMalloc(&p, n * sizeof(T) + 256));
+    HIP_TRY(hipMemset(p, 0, n * sizeof(T) + 256));
+    h->allocs.push_back(p);
+    *ptr = (T *)p;
+    return SMAC_OK;
+}
+
+static void free_handle(smac_handle *h)
+{
+    if (!h) return;
+    for (void *p : h->allocs) (void)hipFree(p);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+static smac_status check_params(const smac_params *p)
+{
+    if (!p) return fail(SMAC_ERR_ARG, "params is NULL");
+    if (p->impsch != 1 && p->impsch != 2)
+        return fail(SMAC_ERR_IMPSCH, "impsch must be 1 (line relaxation) or 2 (LU-SGS)");
+    if (p->iflxo != 3) return fail(SMAC_ERR_IFLXO, "iflxo must be 3 (third-order upwind flux)");
+    if (p->ivis != 0 && p->ivis != 1) return fail(SMAC_ERR_ARG, "ivis must be 0 or 1");
+    if (p->iturb != 0 && p->iturb != 1) return fail(SMAC_ERR_ARG, "iturb must be 0 or 1");
+    if (!(p->underr > 0.0 && p->underr <= 1.0))
+        return fail(SMAC_ERR_ARG, "underr must be in (0, 1] (ref/initia.f:243-247)");
+    if (p->nbcimp < 1) return fail(SMAC_ERR_ARG, "nbcimp must be >= 1");
+    if (p->njsp < 0 || p->nksp < 0 || p->ntjsp < 0) return fail(SMAC_ERR_ARG, "negative sweep count");
+    if (p->ntksp != 0)
+        return fail(SMAC_ERR_ARG, "ntksp > 0 (SA eta sweeps) is not supported; the reference uses 0");
+    if (p->precision != SMAC_FP64)
+        return fail(SMAC_ERR_ARG, "only SMAC_FP64 is available in this build");
+    if (p->line_order != SMAC_ORDER_REFERENCE)
+        return fail(SMAC_ERR_ARG, "only SMAC_ORDER_REFERENCE is available in this build");
+    if (!(p->dtau > 0.0) || !(p->reynum > 0.0) || !(p->beta > 0.0) || !(p->dt > 0.0))
+        return fail(SMAC_ERR_ARG, "dtau, reynum, beta and dt must be positive");
+    return SMAC_OK;
+}
+
+extern "C" {
+
+const char *smac_last_error(void) { return g_err.c_str(); }
+const char *smac_version(void) { return "smac_amd 0.1 (gfx950)"; }
+
+smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *x,
+                        const double *y, const double *rtxy, const double *dj, int nreg,
+                        const smac_bc_region *regs, smac_handle **out)
+{
+    g_err.clear();
+    if (!out) return fail(SMAC_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    smac_status rc = check_params(p);
+    if (rc) return rc;
+    if (!x || !y || !rtxy || !dj || !regs || nreg < 1)
+        return fail(SMAC_ERR_ARG, "grid, metric or BC arrays missing");
+    if (jmax < 5 || kmax < 5 || jmax > 8100 || kmax > 8100)
+        return fail(SMAC_ERR_SIZE, "jmax and kmax must be in [5, 8100] (ref/include/common.f:45)");
+    if ((p->impsch == 1 && (jmax > 2700 || kmax > 2700)) || (p->iturb && jmax > 2000))
+        return fail(SMAC_ERR_SIZE, "reference-order line sweeps stage a line in LDS: "
+                                   "lines longer than 2700 (2000 with SA) are not supported");
+    std::string why;
+    if (!check_gpu(why)) return fail(SMAC_ERR_NOGPU, why);
+    const long N = (long)jmax * kmax;
+    for (long i = 0; i < N; ++i)
+        if (rtxy[i] != 0.0 || rtxy[N + i] != 0.0)
+            return fail(SMAC_ERR_ARG, "time metrics rtxy(:,:,:,1) must be zero (static grid)");
+    smac_handle *h = new smac_handle();
+    h->p = *p;
+    h->d = Dims{jmax, kmax, N};
+    DevParams &dp = h->dp;
+    dp.beta = p->beta; dp.vnu = 1.0 / p->reynum; dp.dtau = p->dtau; dp.dt = p->dt;
+    dp.underr = p->underr; dp.cdis = p->cdis; dp.pout = p->pout; dp.alpha = p->alpha;
+    dp.chord = p->chord; dp.xmom = p->xmom; dp.ymom = p->ymom;
+    dp.impsch = p->impsch; dp.ivis = p->ivis; dp.iturb = p->iturb;
+    // BC regions -> 0-based, with the structure the kernels rely on
+    int wake_idx[2] = {-1, -1}, nwake = 0;
+    for (int i = 0; i < nreg; ++i) {
+        const smac_bc_region &r = regs[i];
+        BcRegion b{r.ibcval, r.jbeg - 1, r.jend - 1, r.kbeg - 1, r.kend - 1};
+        bool jline = b.jbeg == b.jend, kline = b.kbeg == b.kend;
+        if (b.jbeg < 0 || b.jend >= jmax || b.kbeg < 0 || b.kend >= kmax || b.jbeg > b.jend ||
+            b.kbeg > b.kend || !(jline || kline)) {
+            free_handle(h);
+            return fail(SMAC_ERR_BC, "BC region " + std::to_string(i) + " out of range or not a line");
+        }
+        bool ok = (r.ibcval == 31 && jline && (b.jbeg == 0 || b.jbeg == jmax - 1)) ||
+                  (r.ibcval == 0 && kline && b.kbeg == 0) ||
+                  (r.ibcval == 25 && kline && b.kbeg == kmax - 1) ||
+                  (r.ibcval == 60 && kline && b.kbeg == 0);
+        if (!ok) {
+            free_handle(h);
+            return fail(SMAC_ERR_BC, "BC region " + std::to_string(i) + " (ibcval " +
+                                         std::to_string(r.ibcval) +
+                                         ") is not one of the C-grid rows this library implements");
+        }
+        if (r.ibcval == 0) {
+            if (h->walls.n >= 8) { free_handle(h); return fail(SMAC_ERR_BC, "more than 8 walls"); }
+            h->walls.k[h->walls.n] = b.kbeg;
+            h->walls.j1[h->walls.n] = b.jbeg;
+            h->walls.j2[h->walls.n] = b.jend;
+            h->walls.n++;
+        }
+        if (r.ibcval == 60 && nwake < 2) wake_idx[nwake++] = i;
+        h->regs.push_back(b);
+    }
+    std::vector<int> pairs;
+    if (nwake == 2) {
+        const smac_bc_region &a = regs[wake_idx[0]], &b = regs[wake_idx[1]];
+        int na = a.jend - a.jbeg + 1, nb = b.jend - b.jbeg + 1, n = na < nb ? na : nb;
+        for (int i = 0; i < n; ++i) { pairs.push_back(a.jbeg + i - 1); pairs.push_back(b.jend - i - 1); }
+        for (int i = 0; i < n; ++i) { pairs.push_back(b.jbeg + i - 1); pairs.push_back(a.jend - i - 1); }
+    }
+#define TRY(x)                     \
+    do {                           \
+        smac_status r_ = (x);      \
+        if (r_) {                  \
+            std::string e_ = g_err; \
+            free_handle(h);        \
+            g_err = e_;            \
+            return r_;             \
+        }                          \
+    } while (0)
+    (void)hipGetDevice(&h->device);
+    if (hipStreamCreate(&h->stream) != hipSuccess) { free_handle(h); return fail(SMAC_ERR_HIP, "hipStreamCreate failed"); }
+    TRY(dalloc(h, &h->x, N));
+    TRY(dalloc(h, &h->y, N));
+    TRY(dalloc(h, &h->met, 4 * N));
+    TRY(dalloc(h, &h->dj, N));
+    TRY(dalloc(h, &h->q, 3 * N));
+    TRY(dalloc(h, &h->dq, 3 * N));
+    TRY(dalloc(h, &h->s, 3 * N));
+    for (int b = 0; b < 6; ++b) TRY(dalloc(h, &h->blk[b], 9 * N));
+    double **sa_planes[] = {&h->sa.anut, &h->sa.vnut, &h->sa.smin, &h->sa.sjm, &h->sa.sjp,
+                            &h->sa.skm, &h->sa.skp, &h->sa.stc, &h->sa.sf, &h->sa.ds,
+                            &h->sa.vort, &h->sa.cu, &h->sa.ck, &h->sa.cv, &h->sa.fr, &h->sa.bprm};
+    for (double **pp : sa_planes) TRY(dalloc(h, pp, N));
+    TRY(dalloc(h, &h->st, 1));
+    h->w.nred = 512;
+    TRY(dalloc(h, &h->w.red_v, 2 * h->w.nred));
+    TRY(dalloc(h, &h->w.red_i, 2 * h->w.nred));
+    TRY(dalloc(h, &h->w.red_s, h->w.nred));
+    TRY(dalloc(h, &h->w.sa_v, h->w.nred));
+    TRY(dalloc(h, &h->w.flag, 1));
+    if (!pairs.empty()) TRY(dalloc(h, &h->pairs, pairs.size()));
+    h->wake = WakePairs{(int)pairs.size() / 2, h->pairs};
+    // uploads
+    std::vector<double> met(4 * N), q0(3 * N);
+    std::memcpy(met.data(), rtxy + 2 * N, N * sizeof(double));          // xi_x  = rtxy(.,.,1,2)
+    std::memcpy(met.data() + N, rtxy + 4 * N, N * sizeof(double));      // xi_y  = rtxy(.,.,1,3)
+    std::memcpy(met.data() + 2 * N, rtxy + 3 * N, N * sizeof(double));  // eta_x = rtxy(.,.,2,2)
+    std::memcpy(met.data() + 3 * N, rtxy + 5 * N, N * sizeof(double));  // eta_y = rtxy(.,.,2,3)
+    // free-stream initial condition, u = v = 0 on walls (ref/geom.f:88-131)
+    double pi = 4.0 * std::atan(1.0);
+    double ca = std::cos(p->alpha * pi / 180.), sa = std::sin(p->alpha * pi / 180.);
+    for (long i = 0; i < N; ++i) { q0[i] = 1.0; q0[N + i] = ca; q0[2 * N + i] = sa; }
+    for (int wv = 0; wv < h->walls.n; ++wv)
+        for (int j = h->walls.j1[wv]; j <= h->walls.j2[wv]; ++j) {
+            long pnt = (long)j + (long)jmax * h->walls.k[wv];
+            q0[N + pnt] = 0.0;
+            q0[2 * N + pnt] = 0.0;
+        }
+    if (hipMemcpy(h->x, x, N * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->y, y, N * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->dj, dj, N * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->met, met.data(), 4 * N * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->q, q0.data(), 3 * N * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        (!pairs.empty() && hipMemcpy(h->pairs, pairs.data(), pairs.size() * 4,
+                                     hipMemcpyHostToDevice) != hipSuccess)) {
+        free_handle(h);
+        return fail(SMAC_ERR_HIP, "upload failed");
+    }
+    for (auto &e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) { free_handle(h); return fail(SMAC_ERR_HIP, "hipEventCreate failed"); }
+#undef TRY
+    *out = h;
+    return SMAC_OK;
+}
+
+smac_status smac_destroy(smac_handle *h)
+{
+    if (!h) return fail(SMAC_ERR_ARG, "handle is NULL");
+    (void)hipStreamSynchronize(h->stream);
+    free_handle(h);
+    return SMAC_OK;
+}
+
+smac_status smac_set_option(smac_handle *h, const char *name, int value)
+{
+    if (!h || !name) return fail(SMAC_ERR_ARG, "handle or name is NULL");
+    if (!std::strcmp(name, "keep_lhs")) {
+        h->keep_lhs = value != 0;
+        if (h->keep_lhs && !h->lhs_copy[0])
+            for (int b = 0; b < 5; ++b) {
+                smac_status rc = dalloc(h, &h->lhs_copy[b], 9 * h->d.N);
+                if (rc) return rc;
+            }
+        return SMAC_OK;
+    }
+    if (!std::strcmp(name, "timing")) {
+        h->timing = value != 0;
+        return SMAC_OK;
+    }
+    return fail(SMAC_ERR_ARG, std::string("unknown option ") + name);
+}
+
+smac_status smac_set_state(smac_handle *h, const double *q, const double *anut, int nt,
+                           double resmax0, double clift)
+{
+    if (!h) return fail(SMAC_ERR_ARG, "handle is NULL");
+    const long N = h->d.N;
+    if (q) HIP_TRY(hipMemcpy(h->q, q, 3 * N * 8, hipMemcpyHostToDevice));
+    if (anut) HIP_TRY(hipMemcpy(h->sa.anut, anut, N * 8, hipMemcpyHostToDevice));
+    DevStats st;
+    HIP_TRY(hipMemcpy(&st, h->st, sizeof st, hipMemcpyDeviceToHost));
+    st.resmax0 = resmax0;
+    st.clift = clift;
+    HIP_TRY(hipMemcpy(h->st, &st, sizeof st, hipMemcpyHostToDevice));
+    h->nt = nt;
+    if (nt > 0) h->first_sa = false;
+    return SMAC_OK;
+}
+
+smac_status smac_get_state(smac_handle *h, double *q, double *anut, double *vnut)
+{
+    if (!h) return fail(SMAC_ERR_ARG, "handle is NULL");
+    const long N = h->d.N;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (q) HIP_TRY(hipMemcpy(q, h->q, 3 * N * 8, hipMemcpyDeviceToHost));
+    if (anut) HIP_TRY(hipMemcpy(anut, h->sa.anut, N * 8, hipMemcpyDeviceToHost));
+    if (vnut) HIP_TRY(hipMemcpy(vnut, h->sa.vnut, N * 8, hipMemcpyDeviceToHost));
+    return SMAC_OK;
+}
+
+static void mark(smac_handle *h, int i)
+{
+    if (h->timing) (void)hipEventRecord(h->ev[i], h->stream);
+}
+
+// one pass of the time loop body: SA, assembly, BC rows, residual, solve, update + forces
+static smac_status enqueue_pass(smac_handle *h)
+{
+    const Dims &d = h->d;
+    const DevParams &dp = h->dp;
+    hipStream_t sm = h->stream;
+    double *btc = h->blk[0], *bjm = h->blk[1], *bjp = h->blk[2], *bkm = h->blk[3],
+           *bkp = h->blk[4], *scr1 = h->blk[5];
+    h->nt += 1;
+    mark(h, 0);
+    if (dp.ivis == 1 && dp.iturb == 1) {
+        if (h->first_sa) launch_sa_first(sm, d, h->walls, h->x, h->y, h->sa);
+        launch_sa_assemble(sm, d, dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
+                           (int)h->regs.size(), h->regs.data(), h->sa);
+        launch_sa_solve(sm, d, dp, h->p.ntjsp, h->wake, h->sa);
+        launch_sa_update(sm, d, dp, h->w, h->st, h->sa);
+        h->first_sa = false;
+    }
+    mark(h, 1);
+    launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm, bkp);
+    for (const BcRegion &r : h->regs)
+        launch_bc<double>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm, bjp,
+                          bkm, bkp);
+    mark(h, 2);
+    launch_resid(sm, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
+    if (h->keep_lhs)
+        for (int b = 0; b < 5; ++b)
+            HIP_TRY(hipMemcpyAsync(h->lhs_copy[b], h->blk[b], 9 * d.N * 8, hipMemcpyDeviceToDevice, sm));
+    mark(h, 3);
+    if (h->p.impsch == 1)
+        launch_lrelax_ref<double>(sm, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
+                                  h->dq, h->s, btc, bjm, bjp, bkm, bkp);
+    else
+        launch_lusgs3<double>(sm, d, h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, scr1);
+    mark(h, 4);
+    launch_update_force(sm, d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq, h->w, h->st);
+    mark(h, 5);
+    HIP_TRY(hipGetLastError());
+    return SMAC_OK;
+}
+
+static void fill_stats(const smac_handle *h, const DevStats &s, smac_iter_stats *o)
+{
+    o->nt = h->nt;
+    o->jres = s.jres; o->kres = s.kres; o->nonfinite = s.nonfinite;
+    o->resmax = s.resmax; o->resmax0 = s.resmax0; o->divmax = s.divmax; o->turres = s.turres;
+    o->rhsnorm = s.rhsnorm; o->cl = s.cl; o->cd = s.cd; o->cm = s.cm; o->clp = s.clp;
+    o->cdp = s.cdp; o->fxp = s.fxp; o->fyp = s.fyp; o->fxs = s.fxs; o->fys = s.fys;
+    o->fmom = s.fmom;
+}
+
+static void accumulate_timing(smac_handle *h)
+{
+    if (!h->timing) return;
+    float ms;
+    for (int i = 0; i < 5; ++i) {
+        (void)hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]);
+        h->t_acc[i] += ms;
+    }
+    (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[5]);
+    h->t_acc[6] += ms;
+    h->t_n += 1;
+}
+
+smac_status smac_step(smac_handle *h, smac_iter_stats *st)
+{
+    if (!h) return fail(SMAC_ERR_ARG, "handle is NULL");
+    smac_status rc = enqueue_pass(h);
+    if (rc) return rc;
+    DevStats s;
+    HIP_TRY(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    accumulate_timing(h);
+    if (st) fill_stats(h, s, st);
+    if (s.nonfinite) return fail(SMAC_ERR_NONFINITE, "residual or solution went non-finite at nt=" + std::to_string(h->nt));
+    return SMAC_OK;
+}
+
+smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *last, int *done)
+{
+    if (!h) return fail(SMAC_ERR_ARG, "handle is NULL");
+    if (niter < 0) return fail(SMAC_ERR_ARG, "niter < 0");
+    if (h->timing) {
+        for (double &t : h->t_acc) t = 0.0;
+        h->t_n = 0;
+    }
+    double first = -1.0;
+    DevStats s = {};
+    int n = 0;
+    for (; n < niter; ++n) {
+        smac_status rc = enqueue_pass(h);
+        if (rc) return rc;
+        if (drop > 0.0 || h->timing || n == niter - 1) {
+            HIP_TRY(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            accumulate_timing(h);
+            if (s.nonfinite) {
+                if (done) *done = n + 1;
+                if (last) fill_stats(h, s, last);
+                return fail(SMAC_ERR_NONFINITE, "non-finite at nt=" + std::to_string(h->nt));
+            }
+            if (first < 0.0) first = s.rhsnorm;
+            if (drop > 0.0 && s.rhsnorm <= drop * first) { ++n; break; }
+        }
+    }
+    if (n == 0) {
+        HIP_TRY(hipMemcpy(&s, h->st, sizeof s, hipMemcpyDeviceToHost));
+    }
+    if (done) *done = n;
+    if (last) fill_stats(h, s, last);
+    return SMAC_OK;
+}
+
+smac_status smac_timings(smac_handle *h, double out[7])
+{
+    if (!h || !out) return fail(SMAC_ERR_ARG, "NULL argument");
+    for (int i = 0; i < 7; ++i) out[i] = h->t_n ? h->t_acc[i] / h->t_n : 0.0;
+    return SMAC_OK;
+}
+
+smac_status smac_get_array(smac_handle *h, const char *name, double *out)
+{
+    if (!h || !name || !out) return fail(SMAC_ERR_ARG, "NULL argument");
+    const long N = h->d.N;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    struct { const char *n; const double *p; long cnt; } t[] = {
+        {"q", h->q, 3 * N}, {"dq", h->dq, 3 * N}, {"s", h->s, 3 * N}, {"anut", h->sa.anut, N},
+        {"vnut", h->sa.vnut, N}, {"smin", h->sa.smin, N}, {"sf", h->sa.sf, N},
+        {"stc", h->sa.stc, N}, {"ds", h->sa.ds, N}, {"vort", h->sa.vort, N},
+        {"btc", h->lhs_copy[0], 9 * N}, {"bjm", h->lhs_copy[1], 9 * N},
+        {"bjp", h->lhs_copy[2], 9 * N}, {"bkm", h->lhs_copy[3], 9 * N},
+        {"bkp", h->lhs_copy[4], 9 * N},
+    };
+    for (auto &e : t)
+        if (!std::strcmp(e.n, name)) {
+            if (!e.p) return fail(SMAC_ERR_ARG, std::string(name) + " is kept only with option keep_lhs");
+            HIP_TRY(hipMemcpy(out, e.p, e.cnt * 8, hipMemcpyDeviceToHost));
+            return SMAC_OK;
+        }
+    return fail(SMAC_ERR_ARG, std::string("unknown array ") + name);
+}
+
+}  // extern "C"
+
+// ---- Tier 1 -------------------------------------------------------------------------------
+
+struct DevBuf {
+    std::vector<void *> ptrs;
+    ~DevBuf()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    T *get(size_t n)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, n * sizeof(T) + 256) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T *)p;
+    }
+};
+
+static smac_status wake_to_device(DevBuf &b, int jmax, int kmax, int npair, const int *wake,
+                                  WakePairs &w)
+{
+    w = WakePairs{0, nullptr};
+    if (npair <= 0) return SMAC_OK;
+    if (!wake) return fail(SMAC_ERR_ARG, "wake pairs missing");
+    std::vector<int> z(2 * npair);
+    for (int i = 0; i < 2 * npair; ++i) {
+        if (wake[i] < 1 || wake[i] > jmax) return fail(SMAC_ERR_ARG, "wake pair index out of range");
+        z[i] = wake[i] - 1;
+    }
+    (void)kmax;
+    int *d = b.get<int>(2 * npair);
+    if (!d) return fail(SMAC_ERR_HIP, "hipMalloc failed");
+    HIP_TRY(hipMemcpy(d, z.data(), z.size() * 4, hipMemcpyHostToDevice));
+    w = WakePairs{npair, d};
+    return SMAC_OK;
+}
+
+extern "C" {
+
+smac_status smac_lrelax(const smac_params *p, int jmax, int kmax, int npair, const int *wake,
+                        const double *q, double *dq, double *s, const double *btc,
+                        const double *bjm, const double *bjp, const double *bkm,
+                        const double *bkp)
+{
+    g_err.clear();
+    smac_status rc = check_params(p);
+    if (rc) return rc;
+    if (!q || !dq || !s || !btc || !bjm || !bjp || !bkm || !bkp)
+        return fail(SMAC_ERR_ARG, "array argument is NULL");
+    if (jmax < 3 || kmax < 3 || jmax > 2700 || kmax > 2700)
+        return fail(SMAC_ERR_SIZE, "jmax, kmax must be in [3, 2700]");
+    std::string why;
+    if (!check_gpu(why)) return fail(SMAC_ERR_NOGPU, why);
+    const long N = (long)jmax * kmax;
+    Dims d{jmax, kmax, N};
+    DevParams dp = {};
+    dp.underr = p->underr;
+    DevBuf b;
+    double *dq_ = b.get<double>(3 * N), *s_ = b.get<double>(3 * N), *q_ = b.get<double>(3 * N);
+    double *bl[5];
+    const double *hb[5] = {btc, bjm, bjp, bkm, bkp};
+    for (int i = 0; i < 5; ++i) bl[i] = b.get<double>(9 * N);
+    if (!dq_ || !s_ || !q_ || !bl[0] || !bl[1] || !bl[2] || !bl[3] || !bl[4])
+        return fail(SMAC_ERR_HIP, "hipMalloc failed");
+    WakePairs w;
+    rc = wake_to_device(b, jmax, kmax, npair, wake, w);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(q_, q, 3 * N * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s_, s, 3 * N * 8, hipMemcpyHostToDevice));
+    for (int i = 0; i < 5; ++i) HIP_TRY(hipMemcpy(bl[i], hb[i], 9 * N * 8, hipMemcpyHostToDevice));
+    launch_lrelax_ref<double>(0, d, dp, p->njsp, p->nksp, p->nbcimp, w, q_, dq_, s_, bl[0], bl[1],
+                              bl[2], bl[3], bl[4]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(dq, dq_, 3 * N * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(s, s_, 3 * N * 8, hipMemcpyDeviceToHost));
+    return SMAC_OK;
+}
+
+smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int npair,
+                       const int *wake, const double *q, double *dq, double *rhs,
+                       const double *btc, const double *bjm, const double *bjp,
+                       const double *bkm, const double *bkp, double *scr1)
+{
+    g_err.clear();
+    if (!p) return fail(SMAC_ERR_ARG, "params is NULL");
+    if (neqs != 1 && neqs != 3) return fail(SMAC_ERR_NEQS, "neqs must be 1 or 3 (ref/lusgs.f:108-115)");
+    if (!q || !dq || !rhs || !btc || !bjm || !bjp || !bkm || !bkp || !scr1)
+        return fail(SMAC_ERR_ARG, "array argument is NULL");
+    if (jmax < 2 || kmax < 2 || jmax > 8100 || kmax > 8100)
+        return fail(SMAC_ERR_SIZE, "jmax, kmax must be in [2, 8100]");
+    const long N = (long)jmax * kmax;
+    const int nb = neqs * neqs;
+    // the hyperplane schedule needs the couplings the natural ordering reads across rows
+    // to be zero: bjm at j = 1 and bjp at j = jmax
+    for (int e = 0; e < nb; ++e)
+        for (int k = 0; k < kmax; ++k)
+            if (bjm[e * N + (long)jmax * k] != 0.0 || bjp[e * N + (long)jmax * k + jmax - 1] != 0.0)
+                return fail(SMAC_ERR_ARG, "bjm(1,k) and bjp(jmax,k) must be zero (C-grid rows)");
+    std::string why;
+    if (!check_gpu(why)) return fail(SMAC_ERR_NOGPU, why);
+    Dims d{jmax, kmax, N};
+    DevBuf b;
+    double *dq_ = b.get<double>(neqs * N), *r_ = b.get<double>(neqs * N), *q_ = b.get<double>(neqs * N);
+    double *bl[6];
+    const double *hb[5] = {btc, bjm, bjp, bkm, bkp};
+    for (int i = 0; i < 6; ++i) bl[i] = b.get<double>(nb * N);
+    for (int i = 0; i < 6; ++i)
+        if (!bl[i]) return fail(SMAC_ERR_HIP, "hipMalloc failed");
+    if (!dq_ || !r_ || !q_) return fail(SMAC_ERR_HIP, "hipMalloc failed");
+    WakePairs w;
+    smac_status rc = wake_to_device(b, jmax, kmax, npair, wake, w);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(q_, q, neqs * N * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(r_, rhs, neqs * N * 8, hipMemcpyHostToDevice));
+    for (int i = 0; i < 5; ++i) HIP_TRY(hipMemcpy(bl[i], hb[i], nb * N * 8, hipMemcpyHostToDevice));
+    if (neqs == 3)
+        launch_lusgs3<double>(0, d, w, q_, dq_, r_, bl[0], bl[1], bl[2], bl[3], bl[4], bl[5]);
+    else
+        launch_lusgs1(0, d, w, q_, dq_, r_, bl[0], bl[1], bl[2], bl[3], bl[4], bl[5]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(dq, dq_, neqs * N * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rhs, r_, neqs * N * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(scr1, bl[5], nb * N * 8, hipMemcpyDeviceToHost));
+    return SMAC_OK;
+}
+
+}  // extern "C"

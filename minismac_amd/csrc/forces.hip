@@ -1,0 +1,136 @@
+// Solution update and wall forces: update (ref/step.f:760-856), force/fmk
+// (ref/geom.f:135-213, ref/fm.f:156-324) and the force coefficients of iterout with the
+// circulation fed back to the far field (ref/smac2d.f:1709-1741, 1825).
+//
+// The force sums are sequential in the reference (fxp = fxp - fx face after face), and CL
+// enters the next pass's far-field rows, so they are kept in the reference order: the
+// per-face terms are formed in parallel into LDS, then five lanes run the five running sums
+// (fxp, fyp, fxs, fys, fmom) in the reference's order.
+#include "smac_dev.h"
+#include "smac_kernels.h"
+
+namespace smac {
+
+__global__ void k_update(long n3, double *__restrict__ q, const double *__restrict__ dq,
+                         int *__restrict__ flag)
+{
+    int bad = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+         i += (long)gridDim.x * blockDim.x) {
+        double v = q[i] + dq[i];
+        q[i] = v;
+        bad |= !(fabs(v) <= 1.0e300);
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+constexpr int FORCE_THREADS = 256;
+
+__global__ void __launch_bounds__(FORCE_THREADS)
+k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const double *__restrict__ y,
+        const double *__restrict__ met, const double *__restrict__ dj, const double *__restrict__ q,
+        int *__restrict__ flag, DevStats *st)
+{
+    __shared__ double t_fx[FORCE_THREADS], t_fy[FORCE_THREADS], t_m1[FORCE_THREADS],
+        t_m2[FORCE_THREADS];
+    const long N = d.N;
+    const double *r22 = met + 2 * N, *r23 = met + 3 * N;
+    const double x0 = prm.xmom, y0 = prm.ymom, vnu = prm.vnu;
+    double fxp = 0.0, fyp = 0.0, fxs = 0.0, fys = 0.0, fmom = 0.0;   // lanes 0..4 own one each
+    const int lane = threadIdx.x;
+    for (int w = 0; w < wl.n; ++w) {
+        const int k = wl.k[w], jbeg = wl.j1[w], jend = wl.j2[w];
+        fxp = 0.; fyp = 0.; fxs = 0.; fys = 0.;   // fmk starts its sums afresh (ref/fm.f:196-199)
+        for (int pass = 0; pass < (prm.ivis != 0 ? 2 : 1); ++pass) {
+            for (int j0 = jbeg; j0 <= jend - 1; j0 += FORCE_THREADS) {
+                const int j = j0 + lane;
+                if (j <= jend - 1) {
+                    const long p = P(d, j, k), p1 = p + 1;
+                    double da1 = 0.5 * (r22[p1] + r22[p]);
+                    double da2 = 0.5 * (r23[p1] + r23[p]);
+                    double xave = 0.5 * (x[p1] + x[p]) - x0;
+                    double yave = 0.5 * (y[p1] + y[p]) - y0;
+                    if (pass == 0) {
+                        // pressure (ref/fm.f:210-233)
+                        double pave = 0.5 * (q[p1] + q[p]);
+                        double fx = pave * da1, fy = pave * da2;
+                        t_fx[lane] = fx;
+                        t_fy[lane] = fy;
+                        t_m1[lane] = xave * fy - yave * fx;
+                    } else {
+                        // shear from one-sided eta derivatives (ref/fm.f:277-309)
+                        double tt[2][3];
+                        for (int a = 0; a < 2; ++a) {
+                            const long pa = p + a, s = d.jm;
+                            double ueta = 0.5 * (-3. * q[N + pa] + 4. * q[N + pa + s] - q[N + pa + 2 * s]);
+                            double veta = 0.5 * (-3. * q[2 * N + pa] + 4. * q[2 * N + pa + s] -
+                                                 q[2 * N + pa + 2 * s]);
+                            tt[a][0] = 2. * vnu * r22[pa] * dj[pa] * ueta;
+                            tt[a][1] = 2. * vnu * r23[pa] * dj[pa] * veta;
+                            tt[a][2] = vnu * (r23[pa] * ueta + r22[pa] * veta) * dj[pa];
+                        }
+                        double t11ave = 0.5 * (tt[1][0] + tt[0][0]);
+                        double t22ave = 0.5 * (tt[1][1] + tt[0][1]);
+                        double t12ave = 0.5 * (tt[1][2] + tt[0][2]);
+                        double t21ave = t12ave;
+                        double fx = t11ave * da1 + t12ave * da2;
+                        double fy = t21ave * da1 + t22ave * da2;
+                        t_fx[lane] = fx;
+                        t_fy[lane] = fy;
+                        t_m1[lane] = xave * fy;
+                        t_m2[lane] = yave * fx;
+                    }
+                }
+                __syncthreads();
+                const int cnt = (jend - 1) - j0 + 1 < FORCE_THREADS ? (jend - 1) - j0 + 1 : FORCE_THREADS;
+                if (lane < 5) {
+                    for (int i = 0; i < cnt; ++i) {
+                        if (pass == 0) {
+                            if (lane == 0) fxp = fxp - t_fx[i];
+                            else if (lane == 1) fyp = fyp - t_fy[i];
+                            else if (lane == 4) fmom = fmom - t_m1[i];
+                        } else {
+                            if (lane == 2) fxs = fxs + t_fx[i];
+                            else if (lane == 3) fys = fys + t_fy[i];
+                            else if (lane == 4) fmom = fmom + t_m1[i] - t_m2[i];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    __shared__ double res[5];
+    if (lane < 5) res[lane] = lane == 0 ? fxp : lane == 1 ? fyp : lane == 2 ? fxs : lane == 3 ? fys : fmom;
+    __syncthreads();
+    if (lane == 0) {
+        fxp = res[0]; fyp = res[1]; fxs = res[2]; fys = res[3]; fmom = res[4];
+        double pi = 4.0 * atan(1.0);
+        double ca = cos(prm.alpha * pi / 180.0), sa = sin(prm.alpha * pi / 180.0);
+        double chord = prm.chord;
+        double cd = (2. * (fxp + fxs) * ca + 2. * (fyp + fys) * sa) / chord;
+        double cl = (2. * (fyp + fys) * ca - 2. * (fxp + fxs) * sa) / chord;
+        st->cd = cd;
+        st->cl = cl;
+        st->cm = -(2. * fmom / (chord * chord));
+        st->cdp = (2. * fxp * ca + 2. * fyp * sa) / chord;
+        st->clp = (2. * fyp * ca - 2. * fxp * sa) / chord;
+        st->fxp = fxp; st->fyp = fyp; st->fxs = fxs; st->fys = fys; st->fmom = fmom;
+        st->clift = cl;
+        if (*flag) st->nonfinite = 1;
+        *flag = 0;
+    }
+}
+
+void launch_update_force(hipStream_t sm, const Dims &d, const DevParams &prm, const Walls &wl,
+                         const double *x, const double *y, const double *met, const double *dj,
+                         double *q, const double *dq, Scratch &w, DevStats *st)
+{
+    int nb = (int)((3 * d.N + 255) / 256);
+    nb = nb > 4096 ? 4096 : nb;
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, sm, 3 * d.N, q, dq, w.flag);
+    hipLaunchKernelGGL(k_force, dim3(1), dim3(FORCE_THREADS), 0, sm, d, prm, wl, x, y, met, dj, q,
+                       w.flag, st);
+}
+
+}  // namespace smac

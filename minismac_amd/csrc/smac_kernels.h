@@ -1,0 +1,86 @@
+// Host-side launchers of the gfx950 kernels (one translation unit per kernel family).
+#pragma once
+
+#include "smac_dev.h"
+
+namespace smac {
+
+struct BcRegion {
+    int ibcval, jbeg, jend, kbeg, kend;   // 0-based, resolved
+};
+
+// device scratch owned by a handle
+struct Scratch {
+    int nred;            // blocks of the residual reduction
+    double *red_v;       // 2*nred
+    long *red_i;         // 2*nred
+    double *red_s;       // nred
+    double *sa_v;        // nred: SA max(ds) partials
+    int *flag;           // non-finite flag
+};
+
+struct WakePairs {
+    int n;
+    const int *pairs;   // device, 2n ints, 0-based (jt, jb)
+};
+
+struct Walls {
+    int n;              // k-line walls only
+    int k[8], j1[8], j2[8];
+};
+
+// assemble.hip
+template <typename LT>
+void launch_assemble(hipStream_t, const Dims &, const DevParams &, const double *q,
+                     const double *met, const double *dj, double *s, LT *btc, LT *bjm, LT *bjp,
+                     LT *bkm, LT *bkp);
+template <typename LT>
+void launch_bc(hipStream_t, const Dims &, const DevParams &, const BcRegion &, const DevStats *,
+               const double *x, const double *y, const double *met, const double *q, double *s,
+               LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
+void launch_resid(hipStream_t, const Dims &, int nt, double beta, const double *s,
+                  const double *dj, Scratch &, DevStats *);
+
+// linerelax.hip: lrelax in the reference's line order (ref/lrelax.f:51-221)
+template <typename LT>
+void launch_lrelax_ref(hipStream_t, const Dims &, const DevParams &, int njsp, int nksp,
+                       int nbcimp, const WakePairs &, const double *q, double *dq, double *s,
+                       LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
+// zebra order: odd lines then even lines, each colour all at once
+template <typename LT>
+void launch_lrelax_zebra(hipStream_t, const Dims &, const DevParams &, int njsp, int nksp,
+                         int nbcimp, const WakePairs &, const double *q, double *dq, double *s,
+                         LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
+void launch_wake(hipStream_t, const Dims &, int neqs, const WakePairs &, const double *sq,
+                 double *ds, double *rhs);
+
+// lusgs.hip: LU-SGS with hyperplane (j+k = const) sweeps (ref/lusgs.f, ref/ilusol.f)
+template <typename LT>
+void launch_lusgs3(hipStream_t, const Dims &, const WakePairs &, const double *q, double *dq,
+                   double *rhs, const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm,
+                   const LT *bkp, LT *scr1);
+void launch_lusgs1(hipStream_t, const Dims &, const WakePairs &, const double *q, double *dq,
+                   double *rhs, const double *btc, const double *bjm, const double *bjp,
+                   const double *bkm, const double *bkp, double *scr1);
+
+// sa.hip: Spalart-Allmaras (ref/tursa.f, ref/lreltur.f)
+struct SaArrays {
+    double *anut, *vnut, *smin, *sjm, *sjp, *skm, *skp, *stc, *sf, *ds, *vort, *cu, *ck, *cv,
+        *fr, *bprm;
+};
+void launch_sa_first(hipStream_t, const Dims &, const Walls &, const double *x, const double *y,
+                     SaArrays &);
+void launch_sa_assemble(hipStream_t, const Dims &, const DevParams &, int first_pass,
+                        const double *q, const double *met, const double *dj, int nreg,
+                        const BcRegion *regs, SaArrays &);
+void launch_sa_solve(hipStream_t, const Dims &, const DevParams &, int ntjsp, const WakePairs &,
+                     SaArrays &);
+void launch_sa_update(hipStream_t, const Dims &, const DevParams &, Scratch &, DevStats *,
+                      SaArrays &);
+
+// update + wall forces (ref/step.f:760-856; ref/geom.f:135-213, ref/fm.f:156-324)
+void launch_update_force(hipStream_t, const Dims &, const DevParams &, const Walls &,
+                         const double *x, const double *y, const double *met, const double *dj,
+                         double *q, const double *dq, Scratch &, DevStats *);
+
+}  // namespace smac

@@ -1,0 +1,182 @@
+"""Host side of the gfx950 implicit iteration: ctypes binding of libsmac_amd.so.
+
+``Solver`` holds one device-resident case (smac_create/smac_step/smac_run, the replacement
+of the reference's solve loop body around ``step``, ref/smac2d.f:1376-1431); ``lrelax`` and
+``lusgs`` are the host-array drop-ins for the reference's own call sites
+(ref/step.f:538, ref/step.f:552, ref/tursa.f:252).  There is no CPU fallback: if the
+library is missing or no gfx950 device is visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import STATUS_NAMES, SmacIterStats, SmacParams, SmacBcRegion
+from .case import Case, dptr, fortran
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsmac_amd.so")
+
+# every entry point declared in include/smac_amd.h
+EXPORTS = [
+    "smac_create", "smac_destroy", "smac_set_state", "smac_get_state", "smac_step", "smac_run",
+    "smac_get_array", "smac_timings", "smac_set_option", "smac_lrelax", "smac_lusgs",
+    "smac_last_error", "smac_version",
+]
+
+_lib = None
+
+
+class SmacError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        D = P(C.c_double)
+        L.smac_create.argtypes = [P(SmacParams), C.c_int, C.c_int, D, D, D, D, C.c_int,
+                                  P(SmacBcRegion), P(C.c_void_p)]
+        L.smac_destroy.argtypes = [C.c_void_p]
+        L.smac_set_state.argtypes = [C.c_void_p, D, D, C.c_int, C.c_double, C.c_double]
+        L.smac_get_state.argtypes = [C.c_void_p, D, D, D]
+        L.smac_step.argtypes = [C.c_void_p, P(SmacIterStats)]
+        L.smac_run.argtypes = [C.c_void_p, C.c_int, C.c_double, P(SmacIterStats), P(C.c_int)]
+        L.smac_get_array.argtypes = [C.c_void_p, C.c_char_p, D]
+        L.smac_timings.argtypes = [C.c_void_p, D]
+        L.smac_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.smac_lrelax.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, P(C.c_int)] + [D] * 8
+        L.smac_lusgs.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, C.c_int,
+                                 P(C.c_int)] + [D] * 9
+        for fn in EXPORTS[:-2]:
+            getattr(L, fn).restype = C.c_int
+        L.smac_last_error.restype = C.c_char_p
+        L.smac_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise SmacError(rc, where, lib().smac_last_error().decode(errors="replace"))
+
+
+_PLANES = {"q": 3, "dq": 3, "s": 3, "anut": 1, "vnut": 1, "smin": 1, "sf": 1, "stc": 1,
+           "ds": 1, "vort": 1, "btc": 9, "bjm": 9, "bjp": 9, "bkm": 9, "bkp": 9}
+
+
+class Solver:
+    """One case resident on the current HIP device."""
+
+    def __init__(self, case: Case, keep_lhs: bool = False, timing: bool = False):
+        L = lib()
+        self.case = case
+        self.jmax, self.kmax = case.jmax, case.kmax
+        self.params = case.params
+        x, y = fortran(case.x), fortran(case.y)
+        rtxy, dj = case.metrics()
+        rtxy, dj = fortran(rtxy), fortran(dj)
+        self._regs = case.c_regions()
+        h = C.c_void_p()
+        _check(L.smac_create(C.byref(self.params), self.jmax, self.kmax, dptr(x), dptr(y),
+                             dptr(rtxy), dptr(dj), len(self._regs), self._regs, C.byref(h)),
+               "smac_create")
+        self.h = h
+        if keep_lhs:
+            _check(L.smac_set_option(self.h, b"keep_lhs", 1), "smac_set_option")
+        if timing:
+            _check(L.smac_set_option(self.h, b"timing", 1), "smac_set_option")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().smac_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self) -> dict:
+        st = SmacIterStats()
+        _check(lib().smac_step(self.h, C.byref(st)), "smac_step")
+        return st.as_dict()
+
+    def run(self, niter: int, drop: float = 0.0):
+        st = SmacIterStats()
+        done = C.c_int(0)
+        _check(lib().smac_run(self.h, niter, drop, C.byref(st), C.byref(done)), "smac_run")
+        return st.as_dict(), done.value
+
+    def timings(self) -> dict:
+        out = (C.c_double * 7)()
+        _check(lib().smac_timings(self.h, out), "smac_timings")
+        keys = ["sa", "assemble", "resid", "solve", "update_force", "unused", "pass"]
+        return dict(zip(keys, list(out)))
+
+    def array(self, name: str) -> np.ndarray:
+        planes = _PLANES[name]
+        out = np.zeros((self.jmax, self.kmax, planes) if planes > 1 else (self.jmax, self.kmax),
+                       order="F")
+        _check(lib().smac_get_array(self.h, name.encode(), dptr(out)), "smac_get_array")
+        return out
+
+    def set_state(self, q=None, anut=None, nt=0, resmax0=1.0, clift=0.0) -> None:
+        qa = fortran(q) if q is not None else None
+        aa = fortran(anut) if anut is not None else None
+        _check(lib().smac_set_state(self.h, dptr(qa) if qa is not None else None,
+                                    dptr(aa) if aa is not None else None, nt, resmax0, clift),
+               "smac_set_state")
+
+    def get_state(self):
+        q = np.zeros((self.jmax, self.kmax, 3), order="F")
+        a = np.zeros((self.jmax, self.kmax), order="F")
+        v = np.zeros((self.jmax, self.kmax), order="F")
+        _check(lib().smac_get_state(self.h, dptr(q), dptr(a), dptr(v)), "smac_get_state")
+        return q, a, v
+
+
+def _pairs_arg(pairs):
+    if pairs is None or len(pairs) == 0:
+        return 0, None, None
+    arr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1))
+    return arr.size // 2, arr, arr.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def lrelax(params: SmacParams, q, dq, s, btc, bjm, bjp, bkm, bkp, wake_pairs=None) -> None:
+    """Drop-in for lrelax(jmax,kmax,q,dq,s,btc,bjm,bjp,bkm,bkp) (ref/lrelax.f:51): writes
+    dq and the wake-cut rows of s in place (Fortran-ordered float64 arrays)."""
+    jmax, kmax = q.shape[:2]
+    n, keep, ptr = _pairs_arg(wake_pairs)
+    for a in (q, dq, s, btc, bjm, bjp, bkm, bkp):
+        if not (a.dtype == np.float64 and a.flags.f_contiguous):
+            raise TypeError("arrays must be Fortran-ordered float64")
+    _check(lib().smac_lrelax(C.byref(params), jmax, kmax, n, ptr, dptr(q), dptr(dq), dptr(s),
+                             dptr(btc), dptr(bjm), dptr(bjp), dptr(bkm), dptr(bkp)), "smac_lrelax")
+    del keep
+
+
+def lusgs(params: SmacParams, neqs: int, q, dq, rhs, btc, bjm, bjp, bkm, bkp, scr1,
+          wake_pairs=None) -> None:
+    """Drop-in for lusgs(jmax,kmax,neqs,q,dq,rhs,btc,bjm,bjp,bkm,bkp,scr1)
+    (ref/lusgs.f:51): writes dq, rhs and scr1 in place."""
+    jmax, kmax = q.shape[:2]
+    n, keep, ptr = _pairs_arg(wake_pairs)
+    for a in (q, dq, rhs, btc, bjm, bjp, bkm, bkp, scr1):
+        if not (a.dtype == np.float64 and a.flags.f_contiguous):
+            raise TypeError("arrays must be Fortran-ordered float64")
+    _check(lib().smac_lusgs(C.byref(params), jmax, kmax, neqs, n, ptr, dptr(q), dptr(dq),
+                            dptr(rhs), dptr(btc), dptr(bjm), dptr(bjp), dptr(bkm), dptr(bkp),
+                            dptr(scr1)), "smac_lusgs")
+    del keep
