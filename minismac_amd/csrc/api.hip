@@ -141,6 +141,8 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
         return fail(SMAC_ERR_ARG, "grid, metric or BC arrays missing");
     if (jmax < 5 || kmax < 5 || jmax > 8100 || kmax > 8100)
         return fail(SMAC_ERR_SIZE, "jmax and kmax must be in [5, 8100] (ref/include/common.f:45)");
+    if (p->impsch == 2 && jmax > 4096)
+        return fail(SMAC_ERR_SIZE, "LU-SGS column pipeline: jmax > 4096 is not supported");
     if ((p->impsch == 1 && (jmax > 2700 || kmax > 2700)) || (p->iturb && jmax > 2000))
         return fail(SMAC_ERR_SIZE, "reference-order line sweeps stage a line in LDS: "
                                    "lines longer than 2700 (2000 with SA) are not supported");
@@ -355,7 +357,8 @@ static smac_status enqueue_pass(smac_handle *h)
         launch_lrelax_ref<double>(sm, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
                                   h->dq, h->s, btc, bjm, bjp, bkm, bkp);
     else
-        launch_lusgs3<double>(sm, d, h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, scr1);
+        launch_lusgs3<double>(sm, d, h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, nullptr);
+    (void)scr1;
     mark(h, 4);
     launch_update_force(sm, d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq, h->w, h->st);
     mark(h, 5);
@@ -555,8 +558,8 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
     if (neqs != 1 && neqs != 3) return fail(SMAC_ERR_NEQS, "neqs must be 1 or 3 (ref/lusgs.f:108-115)");
     if (!q || !dq || !rhs || !btc || !bjm || !bjp || !bkm || !bkp || !scr1)
         return fail(SMAC_ERR_ARG, "array argument is NULL");
-    if (jmax < 2 || kmax < 2 || jmax > 8100 || kmax > 8100)
-        return fail(SMAC_ERR_SIZE, "jmax, kmax must be in [2, 8100]");
+    if (jmax < 2 || kmax < 2 || jmax > 4096 || kmax > 8100)
+        return fail(SMAC_ERR_SIZE, "jmax must be in [2, 4096] and kmax in [2, 8100]");
     const long N = (long)jmax * kmax;
     const int nb = neqs * neqs;
     // the hyperplane schedule needs the couplings the natural ordering reads across rows

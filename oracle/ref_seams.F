@@ -26,6 +26,7 @@ c alters the reference arithmetic.
       return
       end
 
+#ifndef SEAMS_NO_SOLVERS
       subroutine lrelax(jmax,kmax,q,dq,s,btc,bjm,bjp,bkm,bkp)
 #include "common.f"
       __INTEGER jmax, kmax
@@ -59,6 +60,8 @@ c alters the reference arithmetic.
       endif
       return
       end
+
+#endif
 
       subroutine force(jmax,kmax,x,y,q,rtxy,dj,x0,y0,
      &                 fxp,fyp,fxs,fys,fmom,ioverlap)
