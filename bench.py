@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # algorithmic HBM bytes per grid point of each kernel family, FP64: every array a family
-# must read or write counted once per pass (measurement section)
+# must read or write counted once per pass; zero fills and other reference artefacts excluded
 BYTES_PER_POINT = {
     "solve_lusgs": 144 + 264 + 120 + 264,      # lusini3, forward, D multiply, backward
     "solve_lr": 360 + 864 + 360 + 864,         # xi factor, 2 xi sweeps, eta factor, 2 eta sweeps
@@ -67,7 +67,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path's multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+              "(multi-GPU decomposition is not part of this library)", file=sys.stderr)
         return 2
 
     from minismac_amd.case import config_case
