@@ -29,7 +29,8 @@ typedef enum smac_status {
     SMAC_ERR_BC = 6,           /* unsupported or malformed boundary region                */
     SMAC_ERR_NONFINITE = 7,    /* residual or update produced Inf/NaN                     */
     SMAC_ERR_HIP = 8,          /* a HIP runtime call failed                               */
-    SMAC_ERR_NOGPU = 9         /* no gfx950 device visible                                */
+    SMAC_ERR_NOGPU = 9,        /* no gfx950 device visible                                */
+    SMAC_ERR_STALL = 10        /* an LU-SGS band hand-over wait timed out (bounded spin)  */
 } smac_status;
 
 /* Solver settings: the reference's namelist $input (ref/initia.f:85-89) plus the values its
@@ -123,7 +124,8 @@ smac_status smac_timings(smac_handle *h, double out[7]);
 
 /* Options: "keep_lhs" (1: keep a copy of the five assembled blocks of each pass for
  * smac_get_array), "timing" (1: time kernel families with HIP events on the handle's
- * stream).  No counterpart in the reference (diagnostics). */
+ * stream), "lusgs_rows" (1..64, default 64: grid rows per wavefront band of the LU-SGS
+ * sweeps; a tuning knob, results do not depend on it).  No counterpart in the reference. */
 smac_status smac_set_option(smac_handle *h, const char *name, int value);
 
 /* ---- Tier 1: drop-ins at the reference's own Fortran call sites (host arrays) --------- */

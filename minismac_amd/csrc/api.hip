@@ -46,6 +46,9 @@ struct smac_handle {
     Scratch w = {};
     int *pairs = nullptr;
     WakePairs wake = {};
+    LusgsWs lws = {};          // LU-SGS sheared workspace (impsch = 2)
+    int lusgs_rows = 64;
+    std::vector<void *> lws_allocs;
     Walls walls = {};
     std::vector<BcRegion> regs;
     int nt = 0;
@@ -90,9 +93,41 @@ static smac_status dalloc(smac_handle *h, T **ptr, size_t n)
     return SMAC_OK;
 }
 
+static void free_lusgs_ws(smac_handle *h)
+{
+    for (void *p : h->lws_allocs) (void)hipFree(p);
+    h->lws_allocs.clear();
+    h->lws = LusgsWs{};
+}
+
+// LU-SGS workspace for the mean flow (neqs = 3) and, in the same planes, the SA (neqs = 1)
+static smac_status alloc_lusgs_ws(smac_handle *h, int rows)
+{
+    free_lusgs_ws(h);
+    LusgsWs &w = h->lws;
+    lusgs_ws_shape(h->d.jm, h->d.km, rows, w);
+    void *p[4] = {};
+    size_t sz[4] = {(size_t)lusgs_ws_lhs_bytes(w, 3, 8), (size_t)(3 * w.ns * 8),
+                    (size_t)(3 * w.ns * 8), (size_t)(w.nedge * 8)};
+    for (int i = 0; i < 4; ++i) {
+        HIP_TRY(hipMalloc(&p[i], sz[i] + 256));
+        h->lws_allocs.push_back(p[i]);
+        // the padding of the sheared planes is read (and ignored) by idle lanes: keep it finite
+        HIP_TRY(hipMemset(p[i], 0, sz[i] + 256));
+    }
+    w.lhs = p[0];
+    w.r = (double *)p[1];
+    w.dq = (double *)p[2];
+    w.edge = (unsigned long long *)p[3];
+    w.stall = &h->st->stall;
+    h->lusgs_rows = w.rows;
+    return SMAC_OK;
+}
+
 static void free_handle(smac_handle *h)
 {
     if (!h) return;
+    free_lusgs_ws(h);
     for (void *p : h->allocs) (void)hipFree(p);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
@@ -141,8 +176,6 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
         return fail(SMAC_ERR_ARG, "grid, metric or BC arrays missing");
     if (jmax < 5 || kmax < 5 || jmax > 8100 || kmax > 8100)
         return fail(SMAC_ERR_SIZE, "jmax and kmax must be in [5, 8100] (ref/include/common.f:45)");
-    if (p->impsch == 2 && jmax > 4096)
-        return fail(SMAC_ERR_SIZE, "LU-SGS column pipeline: jmax > 4096 is not supported");
     if ((p->impsch == 1 && (jmax > 2700 || kmax > 2700)) || (p->iturb && jmax > 2000))
         return fail(SMAC_ERR_SIZE, "reference-order line sweeps stage a line in LDS: "
                                    "lines longer than 2700 (2000 with SA) are not supported");
@@ -223,6 +256,7 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
                             &h->sa.vort, &h->sa.cu, &h->sa.ck, &h->sa.cv, &h->sa.fr, &h->sa.bprm};
     for (double **pp : sa_planes) TRY(dalloc(h, pp, N));
     TRY(dalloc(h, &h->st, 1));
+    if (p->impsch == 2) TRY(alloc_lusgs_ws(h, h->lusgs_rows));
     h->w.nred = 512;
     TRY(dalloc(h, &h->w.red_v, 2 * h->w.nred));
     TRY(dalloc(h, &h->w.red_i, 2 * h->w.nred));
@@ -288,6 +322,13 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
         h->timing = value != 0;
         return SMAC_OK;
     }
+    if (!std::strcmp(name, "lusgs_rows")) {
+        if (value < 1 || value > 64) return fail(SMAC_ERR_ARG, "lusgs_rows must be in [1, 64]");
+        h->lusgs_rows = value;
+        if (h->p.impsch != 2) return SMAC_OK;
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return alloc_lusgs_ws(h, value);
+    }
     return fail(SMAC_ERR_ARG, std::string("unknown option ") + name);
 }
 
@@ -338,7 +379,7 @@ static smac_status enqueue_pass(smac_handle *h)
         if (h->first_sa) launch_sa_first(sm, d, h->walls, h->x, h->y, h->sa);
         launch_sa_assemble(sm, d, dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
                            (int)h->regs.size(), h->regs.data(), h->sa);
-        launch_sa_solve(sm, d, dp, h->p.ntjsp, h->wake, h->sa);
+        launch_sa_solve(sm, d, dp, h->p.ntjsp, h->wake, &h->lws, h->sa);
         launch_sa_update(sm, d, dp, h->w, h->st, h->sa);
         h->first_sa = false;
     }
@@ -357,7 +398,8 @@ static smac_status enqueue_pass(smac_handle *h)
         launch_lrelax_ref<double>(sm, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
                                   h->dq, h->s, btc, bjm, bjp, bkm, bkp);
     else
-        launch_lusgs3<double>(sm, d, h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, nullptr);
+        launch_lusgs3<double>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm,
+                              bkp, nullptr, false);
     (void)scr1;
     mark(h, 4);
     launch_update_force(sm, d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq, h->w, h->st);
@@ -399,6 +441,7 @@ smac_status smac_step(smac_handle *h, smac_iter_stats *st)
     HIP_TRY(hipStreamSynchronize(h->stream));
     accumulate_timing(h);
     if (st) fill_stats(h, s, st);
+    if (s.stall) return fail(SMAC_ERR_STALL, "LU-SGS band hand-over timed out at nt=" + std::to_string(h->nt));
     if (s.nonfinite) return fail(SMAC_ERR_NONFINITE, "residual or solution went non-finite at nt=" + std::to_string(h->nt));
     return SMAC_OK;
 }
@@ -421,6 +464,10 @@ smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *la
             HIP_TRY(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
             HIP_TRY(hipStreamSynchronize(h->stream));
             accumulate_timing(h);
+            if (s.stall) {
+                if (done) *done = n + 1;
+                return fail(SMAC_ERR_STALL, "LU-SGS band hand-over timed out at nt=" + std::to_string(h->nt));
+            }
             if (s.nonfinite) {
                 if (done) *done = n + 1;
                 if (last) fill_stats(h, s, last);
@@ -558,8 +605,8 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
     if (neqs != 1 && neqs != 3) return fail(SMAC_ERR_NEQS, "neqs must be 1 or 3 (ref/lusgs.f:108-115)");
     if (!q || !dq || !rhs || !btc || !bjm || !bjp || !bkm || !bkp || !scr1)
         return fail(SMAC_ERR_ARG, "array argument is NULL");
-    if (jmax < 2 || kmax < 2 || jmax > 4096 || kmax > 8100)
-        return fail(SMAC_ERR_SIZE, "jmax must be in [2, 4096] and kmax in [2, 8100]");
+    if (jmax < 2 || kmax < 2 || jmax > 8100 || kmax > 8100)
+        return fail(SMAC_ERR_SIZE, "jmax and kmax must be in [2, 8100] (ref/include/common.f:45)");
     const long N = (long)jmax * kmax;
     const int nb = neqs * neqs;
     // the hyperplane schedule needs the couplings the natural ordering reads across rows
@@ -582,15 +629,30 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
     WakePairs w;
     smac_status rc = wake_to_device(b, jmax, kmax, npair, wake, w);
     if (rc) return rc;
+    LusgsWs ws;
+    lusgs_ws_shape(jmax, kmax, 64, ws);
+    ws.lhs = b.get<char>(lusgs_ws_lhs_bytes(ws, neqs, 8));
+    ws.r = b.get<double>(neqs * ws.ns);
+    ws.dq = b.get<double>(neqs * ws.ns);
+    ws.edge = b.get<unsigned long long>(ws.nedge);
+    ws.stall = b.get<int>(1);
+    if (!ws.lhs || !ws.r || !ws.dq || !ws.edge || !ws.stall) return fail(SMAC_ERR_HIP, "hipMalloc failed");
+    HIP_TRY(hipMemset(ws.lhs, 0, lusgs_ws_lhs_bytes(ws, neqs, 8)));
+    HIP_TRY(hipMemset(ws.r, 0, neqs * ws.ns * 8));
+    HIP_TRY(hipMemset(ws.stall, 0, sizeof(int)));
     HIP_TRY(hipMemcpy(q_, q, neqs * N * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(r_, rhs, neqs * N * 8, hipMemcpyHostToDevice));
     for (int i = 0; i < 5; ++i) HIP_TRY(hipMemcpy(bl[i], hb[i], nb * N * 8, hipMemcpyHostToDevice));
     if (neqs == 3)
-        launch_lusgs3<double>(0, d, w, q_, dq_, r_, bl[0], bl[1], bl[2], bl[3], bl[4], bl[5]);
+        launch_lusgs3<double>(0, d, ws, w, q_, dq_, r_, bl[0], bl[1], bl[2], bl[3], bl[4], bl[5],
+                              true);
     else
-        launch_lusgs1(0, d, w, q_, dq_, r_, bl[0], bl[1], bl[2], bl[3], bl[4], bl[5]);
+        launch_lusgs1(0, d, ws, w, q_, dq_, r_, bl[0], bl[1], bl[2], bl[3], bl[4], bl[5], true);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
+    int stall = 0;
+    HIP_TRY(hipMemcpy(&stall, ws.stall, sizeof(int), hipMemcpyDeviceToHost));
+    if (stall) return fail(SMAC_ERR_STALL, "LU-SGS band hand-over timed out");
     HIP_TRY(hipMemcpy(dq, dq_, neqs * N * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(rhs, r_, neqs * N * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(scr1, bl[5], nb * N * 8, hipMemcpyDeviceToHost));

@@ -414,14 +414,15 @@ void launch_sa_assemble(hipStream_t sm, const Dims &d, const DevParams &prm, int
 }
 
 void launch_sa_solve(hipStream_t sm, const Dims &d, const DevParams &prm, int ntjsp,
-                     const WakePairs &w, SaArrays &a)
+                     const WakePairs &w, const LusgsWs *ws, SaArrays &a)
 {
     if (prm.impsch == 1) {
         hipLaunchKernelGGL(k_sa_lreltur, dim3(1), dim3(SA_THREADS), sizeof(double) * 4 * d.jm, sm,
                            d, ntjsp, w.n, w.pairs, a.anut, a.ds, a.sf, a.sjm, a.sjp, a.skm, a.skp,
                            a.stc);
     } else {
-        launch_lusgs1(sm, d, w, a.anut, a.ds, a.sf, a.stc, a.sjm, a.sjp, a.skm, a.skp, a.bprm);
+        launch_lusgs1(sm, d, *ws, w, a.anut, a.ds, a.sf, a.stc, a.sjm, a.sjp, a.skm, a.skp,
+                      nullptr, false);
     }
 }
 

@@ -31,7 +31,8 @@ struct DevStats {
     double resmax, resmax0, divmax, turres, rhsnorm;
     double cl, cd, cm, clp, cdp, fxp, fyp, fxs, fys, fmom;
     double clift;   // enters the far-field rows of the next pass
-    int jres, kres, nonfinite, pad;
+    int jres, kres, nonfinite;
+    int stall;      // a band hand-over of the LU-SGS sweeps timed out
 };
 
 // solver constants the kernels read (copied from smac_params at set-up)

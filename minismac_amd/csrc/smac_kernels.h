@@ -54,14 +54,33 @@ void launch_lrelax_zebra(hipStream_t, const Dims &, const DevParams &, int njsp,
 void launch_wake(hipStream_t, const Dims &, int neqs, const WakePairs &, const double *sq,
                  double *ds, double *rhs);
 
-// lusgs.hip: LU-SGS with hyperplane (j+k = const) sweeps (ref/lusgs.f, ref/ilusol.f)
+// lusgs.hip: LU-SGS with hyperplane (j+k = const) sweeps (ref/lusgs.f, ref/ilusol.f).
+// Workspace in the sheared layout: point (j,k) at (j+k)*ks + k of every plane.
+struct LusgsWs {
+    int rows;                    // rows per band (one wavefront each), 1..64
+    int nband;                   // bands covering k = 0..km-1
+    int ks;                      // pitch of one hyperplane
+    int H;                       // last hyperplane, jm + km - 2
+    long ns;                     // (H+1)*ks, plane stride
+    long nedge;                  // 2*nband*jm*4 slots
+    void *lhs;                   // 5 block groups (factors, bjm, bkm, bjp, bkp), LT
+    double *r;                   // neqs planes: rhs, forward result, middle result
+    double *dq;                  // neqs planes
+    unsigned long long *edge;    // band hand-over slots (forward half, backward half)
+    int *stall;                  // raised when a hand-over wait times out
+};
+void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &);
+long lusgs_ws_lhs_bytes(const LusgsWs &, int neqs, int lt_bytes);
+// fac (nullable) receives the diagonal factors in the natural layout (the reference's scr1);
+// rhs_out also returns the middle-step rhs in the natural layout, as the reference leaves it
 template <typename LT>
-void launch_lusgs3(hipStream_t, const Dims &, const WakePairs &, const double *q, double *dq,
-                   double *rhs, const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm,
-                   const LT *bkp, LT *scr1);
-void launch_lusgs1(hipStream_t, const Dims &, const WakePairs &, const double *q, double *dq,
-                   double *rhs, const double *btc, const double *bjm, const double *bjp,
-                   const double *bkm, const double *bkp, double *scr1);
+void launch_lusgs3(hipStream_t, const Dims &, const LusgsWs &, const WakePairs &, const double *q,
+                   double *dq, double *rhs, const LT *btc, const LT *bjm, const LT *bjp,
+                   const LT *bkm, const LT *bkp, LT *fac, bool rhs_out);
+void launch_lusgs1(hipStream_t, const Dims &, const LusgsWs &, const WakePairs &, const double *q,
+                   double *dq, double *rhs, const double *btc, const double *bjm,
+                   const double *bjp, const double *bkm, const double *bkp, double *fac,
+                   bool rhs_out);
 
 // sa.hip: Spalart-Allmaras (ref/tursa.f, ref/lreltur.f)
 struct SaArrays {
@@ -74,7 +93,7 @@ void launch_sa_assemble(hipStream_t, const Dims &, const DevParams &, int first_
                         const double *q, const double *met, const double *dj, int nreg,
                         const BcRegion *regs, SaArrays &);
 void launch_sa_solve(hipStream_t, const Dims &, const DevParams &, int ntjsp, const WakePairs &,
-                     SaArrays &);
+                     const LusgsWs *, SaArrays &);
 void launch_sa_update(hipStream_t, const Dims &, const DevParams &, Scratch &, DevStats *,
                       SaArrays &);
 

@@ -76,7 +76,8 @@ _PLANES = {"q": 3, "dq": 3, "s": 3, "anut": 1, "vnut": 1, "smin": 1, "sf": 1, "s
 class Solver:
     """One case resident on the current HIP device."""
 
-    def __init__(self, case: Case, keep_lhs: bool = False, timing: bool = False):
+    def __init__(self, case: Case, keep_lhs: bool = False, timing: bool = False,
+                 options: dict | None = None):
         L = lib()
         self.case = case
         self.jmax, self.kmax = case.jmax, case.kmax
@@ -94,6 +95,11 @@ class Solver:
             _check(L.smac_set_option(self.h, b"keep_lhs", 1), "smac_set_option")
         if timing:
             _check(L.smac_set_option(self.h, b"timing", 1), "smac_set_option")
+        for name, value in (options or {}).items():
+            self.set_option(name, value)
+
+    def set_option(self, name: str, value: int) -> None:
+        _check(lib().smac_set_option(self.h, name.encode(), int(value)), "smac_set_option")
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -64,6 +64,39 @@ def test_gpu_matches_reference_golden(name, impsch):
     assert np.array_equal(q, ref["q_final"])
 
 
+@pytest.mark.parametrize("rows", [16, 7, 1])
+def test_gpu_lusgs_band_split_matches_golden(rows):
+    """LU-SGS with the rows cut into several wavefront bands (2, 5 and 31 bands on the
+    119x31 grid) hands rows between bands through HBM: the 100-pass trajectory must still be
+    the reference's bit for bit."""
+    ref = np.load(os.path.join(GOLDEN, "ref_shipped_lusgs.npz"))
+    case = shipped_case(SHIPPED_XY, SHIPPED_BC, impsch=2)
+    print(f"shipped LU-SGS, lusgs_rows={rows}", flush=True)
+    with Solver(case, options={"lusgs_rows": rows}) as g:
+        for it in range(len(ref["nt"])):
+            st = g.step()
+            for k in ("resmax", "jres", "kres", "divmax", "fxp", "fyp", "fxs", "fys", "fmom"):
+                assert st[k] == ref[k][it], f"pass {it + 1}: {k}"
+        q, _, _ = g.get_state()
+    assert np.array_equal(q, ref["q_final"])
+
+
+def test_gpu_lusgs_band_split_invariance_945x241():
+    """At the 945x241 size the result cannot depend on how the rows are cut into bands:
+    64-row bands (4 bands) and 13-row bands (19 bands) give the same state bit for bit."""
+    case = config_case(1, impsch=2)
+    out = []
+    for rows in (64, 13):
+        print(f"945x241 LU-SGS, lusgs_rows={rows}", flush=True)
+        with Solver(case, options={"lusgs_rows": rows}) as g:
+            st, _ = g.run(3)
+            q, anut, _ = g.get_state()
+        out.append((st, q, anut))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 @pytest.mark.parametrize("impsch", [1, 2])
 @pytest.mark.parametrize("nt", [1, 2, 10, 40, 80])
 def test_gpu_sa_single_pass_from_oracle_state(impsch, nt):
@@ -159,3 +192,5 @@ def test_tier1_dropin_matches_oracle(impsch):
     o.solve()
     assert np.array_equal(dq, o.array("dq"))
     assert np.array_equal(s_gpu, o.array("s"))
+    if impsch == 2:
+        assert np.array_equal(scr1, o.array("scr1"))
