@@ -103,12 +103,16 @@ static void free_lusgs_ws(smac_handle *h)
 // LU-SGS workspace for the mean flow (neqs = 3) and, in the same planes, the SA (neqs = 1)
 static smac_status alloc_lusgs_ws(smac_handle *h, int rows)
 {
+    const int pf3 = h->lws.pf3, pf1 = h->lws.pf1;
     free_lusgs_ws(h);
     LusgsWs &w = h->lws;
+    w.pf3 = pf3;
+    w.pf1 = pf1;
     lusgs_ws_shape(h->d.jm, h->d.km, rows, w);
+    if (!lusgs_ws_fits(w)) return fail(SMAC_ERR_SIZE, "grid too large for the LU-SGS workspace");
     void *p[4] = {};
-    size_t sz[4] = {(size_t)lusgs_ws_lhs_bytes(w, 3, 8), (size_t)(3 * w.ns * 8),
-                    (size_t)(3 * w.ns * 8), (size_t)(w.nedge * 8)};
+    size_t sz[4] = {(size_t)lusgs_ws_lhs_bytes(w, 3), (size_t)lusgs_ws_vec_bytes(w, 3),
+                    (size_t)lusgs_ws_vec_bytes(w, 3), (size_t)(w.nedge * 8)};
     for (int i = 0; i < 4; ++i) {
         HIP_TRY(hipMalloc(&p[i], sz[i] + 256));
         h->lws_allocs.push_back(p[i]);
@@ -320,6 +324,14 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
     }
     if (!std::strcmp(name, "timing")) {
         h->timing = value != 0;
+        return SMAC_OK;
+    }
+    if (!std::strcmp(name, "lusgs_prefetch") || !std::strcmp(name, "lusgs_prefetch_sa")) {
+        const bool mean = !std::strcmp(name, "lusgs_prefetch");
+        const bool ok = value == 1 || value == 2 || value == 3 || value == 4 || value == 6 ||
+                        value == 8;
+        if (!ok) return fail(SMAC_ERR_ARG, std::string(name) + ": unsupported prefetch distance");
+        (mean ? h->lws.pf3 : h->lws.pf1) = value;
         return SMAC_OK;
     }
     if (!std::strcmp(name, "lusgs_rows")) {
@@ -629,16 +641,21 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
     WakePairs w;
     smac_status rc = wake_to_device(b, jmax, kmax, npair, wake, w);
     if (rc) return rc;
-    LusgsWs ws;
+    LusgsWs ws{};
     lusgs_ws_shape(jmax, kmax, 64, ws);
-    ws.lhs = b.get<char>(lusgs_ws_lhs_bytes(ws, neqs, 8));
-    ws.r = b.get<double>(neqs * ws.ns);
-    ws.dq = b.get<double>(neqs * ws.ns);
+    if (!lusgs_ws_fits(ws)) return fail(SMAC_ERR_SIZE, "grid too large for the LU-SGS workspace");
+    const long lb = lusgs_ws_lhs_bytes(ws, neqs), vb = lusgs_ws_vec_bytes(ws, neqs);
+    ws.lhs = b.get<char>(lb);
+    ws.r = (double *)b.get<char>(vb);
+    ws.dq = (double *)b.get<char>(vb);
     ws.edge = b.get<unsigned long long>(ws.nedge);
     ws.stall = b.get<int>(1);
     if (!ws.lhs || !ws.r || !ws.dq || !ws.edge || !ws.stall) return fail(SMAC_ERR_HIP, "hipMalloc failed");
-    HIP_TRY(hipMemset(ws.lhs, 0, lusgs_ws_lhs_bytes(ws, neqs, 8)));
-    HIP_TRY(hipMemset(ws.r, 0, neqs * ws.ns * 8));
+    // padding positions are read by idle lanes and must hold finite values
+    HIP_TRY(hipMemset(ws.lhs, 0, lb));
+    HIP_TRY(hipMemset(ws.r, 0, vb));
+    HIP_TRY(hipMemset(ws.dq, 0, vb));
+    HIP_TRY(hipMemset(ws.edge, 0, ws.nedge * 8));
     HIP_TRY(hipMemset(ws.stall, 0, sizeof(int)));
     HIP_TRY(hipMemcpy(q_, q, neqs * N * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(r_, rhs, neqs * N * 8, hipMemcpyHostToDevice));

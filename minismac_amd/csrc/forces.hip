@@ -5,7 +5,7 @@
 // The force sums are sequential in the reference (fxp = fxp - fx face after face), and CL
 // enters the next pass's far-field rows, so they are kept in the reference order: the
 // per-face terms are formed in parallel into LDS, then five lanes run the five running sums
-// (fxp, fyp, fxs, fys, fmom) in the reference's order.
+// (fxp, fyp, fxs, fys, fmom) in the reference's order, side by side.
 #include "smac_dev.h"
 #include "smac_kernels.h"
 
@@ -32,12 +32,13 @@ k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const dou
         int *__restrict__ flag, DevStats *st)
 {
     __shared__ double t_fx[FORCE_THREADS], t_fy[FORCE_THREADS], t_m1[FORCE_THREADS],
-        t_m2[FORCE_THREADS];
+        t_m2[FORCE_THREADS], t_zero[FORCE_THREADS];
     const long N = d.N;
     const double *r22 = met + 2 * N, *r23 = met + 3 * N;
     const double x0 = prm.xmom, y0 = prm.ymom, vnu = prm.vnu;
     double fxp = 0.0, fyp = 0.0, fxs = 0.0, fys = 0.0, fmom = 0.0;   // lanes 0..4 own one each
     const int lane = threadIdx.x;
+    t_zero[lane] = 0.0;
     for (int w = 0; w < wl.n; ++w) {
         const int k = wl.k[w], jbeg = wl.j1[w], jend = wl.j2[w];
         fxp = 0.; fyp = 0.; fxs = 0.; fys = 0.;   // fmk starts its sums afresh (ref/fm.f:196-199)
@@ -83,18 +84,25 @@ k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const dou
                 }
                 __syncthreads();
                 const int cnt = (jend - 1) - j0 + 1 < FORCE_THREADS ? (jend - 1) - j0 + 1 : FORCE_THREADS;
+                // lanes 0..4 run their sums with one instruction stream: lane-selected
+                // operands, acc + (-1)*a == acc - a and acc - (+0) == acc exactly
                 if (lane < 5) {
-                    for (int i = 0; i < cnt; ++i) {
-                        if (pass == 0) {
-                            if (lane == 0) fxp = fxp - t_fx[i];
-                            else if (lane == 1) fyp = fyp - t_fy[i];
-                            else if (lane == 4) fmom = fmom - t_m1[i];
-                        } else {
-                            if (lane == 2) fxs = fxs + t_fx[i];
-                            else if (lane == 3) fys = fys + t_fy[i];
-                            else if (lane == 4) fmom = fmom + t_m1[i] - t_m2[i];
-                        }
+                    const bool act = pass == 0 ? (lane == 0 || lane == 1 || lane == 4) : lane >= 2;
+                    const double *A = (lane == 0 || lane == 2) ? t_fx
+                                      : (lane == 1 || lane == 3) ? t_fy : t_m1;
+                    const double *B = (pass == 1 && lane == 4) ? t_m2 : t_zero;
+                    const double sgn = pass == 0 ? -1.0 : 1.0;
+                    double acc = lane == 0 ? fxp : lane == 1 ? fyp : lane == 2 ? fxs
+                               : lane == 3 ? fys : fmom;
+                    if (act) {
+#pragma unroll 8
+                        for (int i = 0; i < cnt; ++i) acc = (acc + sgn * A[i]) - B[i];
                     }
+                    if (lane == 0) fxp = acc;
+                    else if (lane == 1) fyp = acc;
+                    else if (lane == 2) fxs = acc;
+                    else if (lane == 3) fys = acc;
+                    else fmom = acc;
                 }
                 __syncthreads();
             }

@@ -10,21 +10,26 @@
 //
 // Schedule on gfx950:
 //  1. k_shear: the five block planes and the right-hand side are transposed, through LDS
-//     tiles, into the sheared layout of the workspace, where point (j,k) sits at h*ks + k.
-//     The diagonal factors (lusini3 / lusini1) are formed on the way (fully parallel).
-//  2. k_fwd_band: the rows are cut into bands of `rows` <= 64 rows, one wavefront each.  Lane
-//     l of band b owns row k = b*rows + l and walks h = j+k upward: (j-1,k) is its own result
-//     of the previous step (registers), (j,k-1) is lane l-1's result of the previous step (a
-//     cross-lane shuffle), so a step needs no barrier.  Loads of step h+PF are issued before
-//     step h is computed.  The last row of band b hands its results to band b+1 through an
-//     edge buffer in HBM whose slots start as a signalling-NaN sentinel (no arithmetic
-//     produces it); the consumer spins on a relaxed agent-scope load until the slot is filled,
-//     with a bounded spin count that raises a stall flag instead of hanging.
+//     tiles, into the sheared layout of the workspace: point (j,k) sits at position
+//     (h + HP0)*ks + k, with HP0 empty hyperplanes on either side so that a sweep can run past
+//     its ends without bounds tests.  Planes are stored in pairs (16 bytes per position) so
+//     that one load fetches two of them.  The diagonal factors (lusini3 / lusini1) are formed
+//     on the way (fully parallel).
+//  2. k_band (forward): the rows are cut into bands of `rows` <= 64 rows, one wavefront each.
+//     Lane l of band b owns row k = b*rows + l and walks h = j+k upward: (j-1,k) is its own
+//     result of the previous step (registers), (j,k-1) is lane l-1's result of the previous
+//     step (a DPP wave shift), so a step has no barrier and no LDS traffic.  Operands are
+//     fetched with buffer loads (per-lane offset in one VGPR, plane pair in SGPRs) PF steps
+//     ahead.  The last row of band b hands its results to band b+1 through an edge buffer in
+//     HBM whose slots start as a signalling-NaN sentinel (no arithmetic produces it); the
+//     consumer spins on a relaxed agent-scope load until the slot is filled, with a bounded
+//     spin count that raises a stall flag instead of hanging.
 //  3. k_mid: the D multiply of the middle step (pointwise, fully parallel).
-//  4. k_bwd_band: the mirror image, h downward, bands handing their first row down.
+//  4. k_band (backward): the mirror image, h downward, bands handing their first row down.
 //  5. k_unshear: dq (and, for callers that want it, the middle-step rhs) back to the natural
 //     layout; then the wake-cut rows (bcimpds) as in the reference.
-// Every point performs the reference's expressions in the reference's order.
+// Every point performs the reference's expressions in the reference's order; terms the
+// reference does not form at the first/last row or column are dropped by selects.
 #include "smac_dev.h"
 #include "smac_kernels.h"
 
@@ -33,8 +38,22 @@ namespace smac {
 constexpr unsigned long long EDGE_EMPTY = 0x7ff4dead0000beefull;   // signalling NaN
 constexpr int EDGE_SPIN_MAX = 200000;
 constexpr int BAND_LANES = 64;
+constexpr int HP0 = 16;          // empty hyperplanes before h = 0 and after h = H
+constexpr int PF_MAX = 8;        // 2*PF_MAX - 1 <= HP0
+constexpr int EPAD = 128;        // padding slots on either side of a band's edge row
 
-SMAC_DEV long SX(const LusgsWs &w, int j, int k) { return (long)(j + k) * w.ks + k; }
+// position of point (j,k) in the sheared planes
+SMAC_DEV long SX(const LusgsWs &w, int j, int k) { return (long)(j + k + HP0) * w.ks + k; }
+
+// planes per pair (2 for the 3x3 system, 1 for the scalar one) and element index of plane m
+template <int NEQ> constexpr int pair_width() { return NEQ == 3 ? 2 : 1; }
+template <int NEQ> constexpr int pairs_of(int planes) { return (planes + pair_width<NEQ>() - 1) / pair_width<NEQ>(); }
+template <int NEQ>
+SMAC_DEV long EI(long ns, int m, long pos)
+{
+    constexpr int PW = pair_width<NEQ>();
+    return ((long)(m / PW) * ns + pos) * PW + (m % PW);
+}
 
 // ------------------------------------------------------------------------------------------
 // diagonal factors
@@ -71,9 +90,9 @@ SMAC_DEV void lu_solve3(const AT f[9], AT r1, AT r2, AT r3, AT o[3])
 
 constexpr int TJ = 32, TK = 16, TPAD = TJ + 2, TILE_THREADS = 256;
 
-template <int NP>
+template <typename LT, int NEQ, int NP>
 SMAC_DEV void tile_to_sheared(const LusgsWs &w, int jm, int km, int j0, int k0,
-                              double (*lds)[TK][TPAD], double *dst, long stride)
+                              double (*lds)[TK][TPAD], LT *dst)
 {
     for (int e = threadIdx.x; e < (TJ + TK - 1) * TK; e += TILE_THREADS) {
         const int kk = e % TK, hh = e / TK, jj = hh - kk;
@@ -81,21 +100,7 @@ SMAC_DEV void tile_to_sheared(const LusgsWs &w, int jm, int km, int j0, int k0,
         if (jj < 0 || jj >= TJ || j >= jm || k >= km) continue;
         const long o = SX(w, j, k);
 #pragma unroll
-        for (int m = 0; m < NP; ++m) dst[m * stride + o] = lds[m][kk][jj];
-    }
-}
-
-template <typename LT, int NP>
-SMAC_DEV void tile_to_sheared_lt(const LusgsWs &w, int jm, int km, int j0, int k0,
-                                 double (*lds)[TK][TPAD], LT *dst, long stride)
-{
-    for (int e = threadIdx.x; e < (TJ + TK - 1) * TK; e += TILE_THREADS) {
-        const int kk = e % TK, hh = e / TK, jj = hh - kk;
-        const int j = j0 + jj, k = k0 + kk;
-        if (jj < 0 || jj >= TJ || j >= jm || k >= km) continue;
-        const long o = SX(w, j, k);
-#pragma unroll
-        for (int m = 0; m < NP; ++m) dst[m * stride + o] = (LT)lds[m][kk][jj];
+        for (int m = 0; m < NP; ++m) dst[EI<NEQ>(w.ns, m, o)] = (LT)lds[m][kk][jj];
     }
 }
 
@@ -121,14 +126,23 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         const LT *__restrict__ bjm, const LT *__restrict__ bkm, const LT *__restrict__ bjp,
         const LT *__restrict__ bkp, LT *__restrict__ fac)
 {
-    constexpr int NB = NEQ * NEQ;
+    constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ>();
     __shared__ double lds[NB][TK][TPAD];
     const int j0 = blockIdx.x * TJ, k0 = blockIdx.y * TK;
     LT *wl = (LT *)w.lhs;
-    const long lstride = w.ns;
+    const long gstride = (long)pairs_of<NEQ>(NB) * w.ns * PW;   // elements per block group
 
-    if (blockIdx.x == 0 && blockIdx.y == 0)
-        for (long i = threadIdx.x; i < w.nedge; i += TILE_THREADS) w.edge[i] = EDGE_EMPTY;
+    {
+        // hand-over slots of both sweeps: the jm real ones become the sentinel, the padding
+        // on either side stays a finite value (the dummy region after them is not touched)
+        const long nthr = (long)gridDim.x * gridDim.y * TILE_THREADS;
+        const long t0 = ((long)blockIdx.y * gridDim.x + blockIdx.x) * TILE_THREADS + threadIdx.x;
+        const long nreal = 2L * w.nband * w.epitch;
+        for (long i = t0; i < nreal; i += nthr) {
+            const long pos = (i % w.epitch) / 4;
+            w.edge[i] = (pos >= EPAD && pos < EPAD + d.jm) ? EDGE_EMPTY : 0ull;
+        }
+    }
 
     // group 0: diagonal factors
     for (int e = threadIdx.x; e < TK * TJ; e += TILE_THREADS) {
@@ -150,7 +164,7 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         }
     }
     __syncthreads();
-    tile_to_sheared_lt<LT, NB>(w, d.jm, d.km, j0, k0, lds, wl, lstride);
+    tile_to_sheared<LT, NEQ, NB>(w, d.jm, d.km, j0, k0, lds, wl);
 
     const LT *src[4] = {bjm, bkm, bjp, bkp};
 #pragma unroll
@@ -158,20 +172,20 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         __syncthreads();
         tile_from_natural<LT, NB>(d, j0, k0, lds, src[g]);
         __syncthreads();
-        tile_to_sheared_lt<LT, NB>(w, d.jm, d.km, j0, k0, lds, wl + (long)(g + 1) * NB * lstride,
-                                   lstride);
+        tile_to_sheared<LT, NEQ, NB>(w, d.jm, d.km, j0, k0, lds, wl + (g + 1) * gstride);
     }
     __syncthreads();
     tile_from_natural<double, NEQ>(d, j0, k0, lds, rhs);
     __syncthreads();
-    tile_to_sheared<NEQ>(w, d.jm, d.km, j0, k0, lds, w.r, lstride);
+    tile_to_sheared<double, NEQ, NEQ>(w, d.jm, d.km, j0, k0, lds, w.r);
 }
 
-// sheared -> natural for NP planes
-template <int NP>
+// sheared -> natural for the NEQ planes of a vector (dq, or the middle-step rhs)
+template <int NEQ>
 __global__ void __launch_bounds__(TILE_THREADS)
 k_unshear(Dims d, LusgsWs w, const double *__restrict__ src, double *__restrict__ dst)
 {
+    constexpr int NP = NEQ;
     __shared__ double lds[NP][TK][TPAD];
     const int j0 = blockIdx.x * TJ, k0 = blockIdx.y * TK;
     for (int e = threadIdx.x; e < (TJ + TK - 1) * TK; e += TILE_THREADS) {
@@ -180,7 +194,7 @@ k_unshear(Dims d, LusgsWs w, const double *__restrict__ src, double *__restrict_
         if (jj < 0 || jj >= TJ || j >= d.jm || k >= d.km) continue;
         const long o = SX(w, j, k);
 #pragma unroll
-        for (int m = 0; m < NP; ++m) lds[m][kk][jj] = src[m * w.ns + o];
+        for (int m = 0; m < NP; ++m) lds[m][kk][jj] = src[EI<NEQ>(w.ns, m, o)];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < TK * TJ; e += TILE_THREADS) {
@@ -201,15 +215,17 @@ __global__ void k_mid(Dims d, LusgsWs w)
 {
     constexpr int NB = NEQ * NEQ;
     const LT *F = (const LT *)w.lhs;
+    double *R = w.r;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < w.ns;
          i += (long)gridDim.x * blockDim.x) {
-        const int h = (int)(i / w.ks), k = (int)(i % w.ks), j = h - k;
+        const int h = (int)(i / w.ks) - HP0, k = (int)(i % w.ks), j = h - k;
         if (k >= d.km || j < 0 || j >= d.jm) continue;
         if constexpr (NEQ == 3) {
             double f[NB];
 #pragma unroll
-            for (int m = 0; m < NB; ++m) f[m] = (double)F[m * w.ns + i];
-            const double o0 = w.r[i], o1 = w.r[w.ns + i], o2 = w.r[2 * w.ns + i];
+            for (int m = 0; m < NB; ++m) f[m] = (double)F[EI<NEQ>(w.ns, m, i)];
+            const long i0 = EI<NEQ>(w.ns, 0, i), i1 = EI<NEQ>(w.ns, 1, i), i2 = EI<NEQ>(w.ns, 2, i);
+            const double o0 = R[i0], o1 = R[i1], o2 = R[i2];
             double b11 = 1. / f[0];
             double b12 = f[3] * b11;
             double b13 = f[6] * b11;
@@ -219,11 +235,11 @@ __global__ void k_mid(Dims d, LusgsWs w)
             double b31 = f[2];
             double b32 = f[5] + f[2] * f[3];
             double b33 = 1. / f[8] + f[2] * f[6] + f[5] * f[7];
-            w.r[i] = b11 * o0 + b12 * o1 + b13 * o2;
-            w.r[w.ns + i] = b21 * o0 + b22 * o1 + b23 * o2;
-            w.r[2 * w.ns + i] = b31 * o0 + b32 * o1 + b33 * o2;
+            R[i0] = b11 * o0 + b12 * o1 + b13 * o2;
+            R[i1] = b21 * o0 + b22 * o1 + b23 * o2;
+            R[i2] = b31 * o0 + b32 * o1 + b33 * o2;
         } else {
-            w.r[i] = w.r[i] / (double)F[i];
+            R[i] = R[i] / (double)F[i];
         }
     }
 }
@@ -231,15 +247,94 @@ __global__ void k_mid(Dims d, LusgsWs w)
 // ------------------------------------------------------------------------------------------
 // band sweeps
 
-SMAC_DEV unsigned long long edge_ld(const unsigned long long *p)
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+struct D2 {
+    double x, y;
+};
+
+// raw buffer over `bytes` bytes (gfx9 resource word 3); accesses beyond it read 0 and drop
+SMAC_DEV rsrc_t mk_rsrc(const void *p, unsigned bytes)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+SMAC_DEV D2 bld2(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0));
+}
+SMAC_DEV double bld1(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+}
+SMAC_DEV void bst2(double x, double y, rsrc_t r, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, (D2{x, y})), r, (int)vo,
+                                           (int)so, 0);
+}
+SMAC_DEV void bst1(double x, rsrc_t r, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, x), r, (int)vo, (int)so, 0);
 }
 
-SMAC_DEV void edge_st(unsigned long long *p, double v)
+// NP planes of one group at the current position (vo), pair p at soffset p*pstride
+template <int NEQ, int NP>
+SMAC_DEV void ld_group(double *out, rsrc_t r, unsigned vo, unsigned pstride)
 {
-    __hip_atomic_store(p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (pair_width<NEQ>() == 2) {
+#pragma unroll
+        for (int p = 0; p < NP / 2; ++p) {
+            D2 t = bld2(r, vo, p * pstride);
+            out[2 * p] = t.x;
+            out[2 * p + 1] = t.y;
+        }
+        if constexpr (NP % 2) out[NP - 1] = bld1(r, vo, (NP / 2) * pstride);
+    } else {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) out[p] = bld1(r, vo, p * pstride);
+    }
+}
+
+template <int NEQ>
+SMAC_DEV void st_vec(const double *v, rsrc_t r, unsigned vo, unsigned pstride)
+{
+    if constexpr (NEQ == 3) {
+        bst2(v[0], v[1], r, vo, 0);
+        bst1(v[2], r, vo, pstride);
+    } else {
+        bst1(v[0], r, vo, 0);
+    }
+}
+
+// Hand-over slots are read and written with the agent-coherent cache policy (sc1), the form
+// the compiler gives relaxed agent-scope atomics; every slot is an aligned 8-byte word.
+constexpr int EDGE_POLICY = 0x10;
+
+SMAC_DEV unsigned long long edge_ld(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(unsigned long long,
+                              __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, EDGE_POLICY));
+}
+
+SMAC_DEV void edge_st(double v, rsrc_t r, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, (int)vo, (int)so,
+                                          EDGE_POLICY);
+}
+
+// Re-reads one hand-over slot and waits for it.  Written as one asm block so that the
+// compiler's wait-count bookkeeping never sees a load on the (rare) polling path: a load
+// there, with the full wait it needs, would otherwise degrade the waits the prefetch ring of
+// the sweep relies on.  Same instruction and cache policy as a relaxed agent-scope atomic load.
+SMAC_DEV unsigned long long edge_reload(const unsigned long long *p)
+{
+    unsigned long long v;
+    asm volatile("global_load_dwordx2 %0, %1, off sc1\n\ts_waitcnt vmcnt(0)"
+                 : "=v"(v)
+                 : "v"(p)
+                 : "memory");
+    return v;
 }
 
 // waits (bounded) until the NEQ slots at p are filled; on time-out raises the stall flag and
@@ -262,229 +357,239 @@ SMAC_DEV void edge_wait(const unsigned long long *p, unsigned long long e[NEQ], 
         }
         __builtin_amdgcn_s_sleep(2);
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) e[m] = edge_ld(p + m);
+        for (int m = 0; m < NEQ; ++m) e[m] = edge_reload(p + m);
     }
+}
+
+// lane l receives lane l-1's value (FWD) or lane l+1's (backward); the end lane receives 0
+template <bool FWD>
+SMAC_DEV double wave_shift(double v)
+{
+    constexpr int ctrl = FWD ? 0x138 : 0x130;   // DPP wave_shr:1 / wave_shl:1, bound_ctrl
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, ctrl, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), ctrl, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+// Forces x to be computed where it stands: a term that is formed and then selected stays a
+// select instead of being sunk into a divergent branch around its computation.
+SMAC_DEV double keep(double x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
 }
 
 // Per-step operands of one lane, loaded PF steps ahead.
-template <typename LT, int NEQ>
+template <int NEQ>
 struct Slot {
-    LT f[NEQ * NEQ], a[NEQ * NEQ], b[NEQ * NEQ];
-    double r[NEQ];
+    double f[NEQ * NEQ], jn[NEQ * NEQ], kn[NEQ * NEQ], r[NEQ];
     unsigned long long e[NEQ];
 };
 
-template <typename LT, int NEQ>
-SMAC_DEV void slot_load(Slot<LT, NEQ> &s, const LusgsWs &w, const LT *F, const LT *A, const LT *B,
-                        const double *R, int h, int k, bool edge_lane, const unsigned long long *ep,
-                        int j, int jm)
-{
-    constexpr int NB = NEQ * NEQ;
-    const int hc = h < w.H ? (h > 0 ? h : 0) : w.H;
-    const long i = (long)hc * w.ks + k;
+// One wavefront band of one sweep of ilusol3 / ilusol1.  FWD: the forward part
+// (ref/ilusol.f:127-202 and 502-540) from the rhs planes into the rhs planes.  Backward:
+// ref/ilusol.f:353-469 and 560-591, from the middle-step rhs into the dq planes.
+// j-neighbour block: bjm (FWD) / bjp; k-neighbour block: bkm (FWD) / bkp.
+template <int NEQ, int PF, bool FWD>
+struct Band {
+    static constexpr int NB = NEQ * NEQ;
+    Dims d;
+    int k, kin, kout;
+    bool lane_in, kterm, broken;
+    unsigned vout;      // producer lane: 0; others: out of range, so their stores drop
+    const unsigned long long *ein_ptr;
+    int *stall;
+    rsrc_t rF, rJ, rK, rR, rO, rEin, rEout;
+    unsigned pstride;
+    int dstep;
+    // running state
+    double own[NEQ];
+    Slot<NEQ> s[PF];
+    int h;
+    unsigned vo, vp;       // byte offset of this lane's position: current / prefetch step
+    unsigned ein_pf, eout; // soffset of the hand-over slots: prefetch step / current step
+
+    SMAC_DEV void load(Slot<NEQ> &sl, unsigned v, unsigned so)
+    {
+        ld_group<NEQ, NB>(sl.f, rF, v, pstride);
+        ld_group<NEQ, NB>(sl.jn, rJ, v, pstride);
+        ld_group<NEQ, NB>(sl.kn, rK, v, pstride);
+        ld_group<NEQ, NEQ>(sl.r, rR, v, pstride);
+        // padded: rows outside 0..jm-1 read finite padding, bands without a neighbour read
+        // a region of zeros
 #pragma unroll
-    for (int m = 0; m < NB; ++m) {
-        s.f[m] = F[m * w.ns + i];
-        s.a[m] = A[m * w.ns + i];
-        s.b[m] = B[m * w.ns + i];
+        for (int m = 0; m < NEQ; ++m) sl.e[m] = edge_ld(rEin, m * 8, so);
     }
+
+    // JSEL: some lane may sit on the first (FWD) / last column, where the reference does not
+    // form the j-neighbour term; KSEL: the band holds the first (FWD) / last grid row, where it
+    // does not form the k-neighbour term.  Elsewhere both terms are formed unconditionally.
+    template <bool JSEL, bool KSEL>
+    SMAC_DEV void step(Slot<NEQ> &c)
+    {
+        double nb[NEQ];
 #pragma unroll
-    for (int m = 0; m < NEQ; ++m) s.r[m] = R[m * w.ns + i];
-    if (edge_lane && j >= 0 && j < jm) {
+        for (int m = 0; m < NEQ; ++m) nb[m] = wave_shift<FWD>(own[m]);
+        bool empty = false;
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) s.e[m] = edge_ld(ep + (long)j * 4 + m);
+        for (int m = 0; m < NEQ; ++m) empty = empty || c.e[m] == EDGE_EMPTY;
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(empty) != 0, 0))   // uniform; rare
+            edge_wait<NEQ>(ein_ptr + (long)(h - kin + EPAD) * 4, c.e, broken, stall);
+#pragma unroll
+        for (int m = 0; m < NEQ; ++m)
+            nb[m] = lane_in ? __longlong_as_double((long long)c.e[m]) : nb[m];
+        const int j = h - k;
+        // FWD: no j-term at j = 1 (bjm(1,k)*rhs(jmax,k-1) is an exact zero); backward: none at
+        // j = jmax (bjp(jmax,k)*dq(1,k+1), likewise)
+        const bool jterm = FWD ? j > 0 : j < d.jm - 1;
+        double o[NEQ];
+        if constexpr (NEQ == 3) {
+            double r[3] = {c.r[0], c.r[1], c.r[2]};
+            auto jpart = [&]() {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const double t = r[m] - c.jn[m] * own[0] - c.jn[m + 3] * own[1] -
+                                     c.jn[m + 6] * own[2];
+                    r[m] = JSEL ? (jterm ? keep(t) : r[m]) : t;
+                }
+            };
+            auto kpart = [&]() {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const double t = r[m] - c.kn[m] * nb[0] - c.kn[m + 3] * nb[1] -
+                                     c.kn[m + 6] * nb[2];
+                    r[m] = KSEL ? (kterm ? keep(t) : r[m]) : t;
+                }
+            };
+            if constexpr (FWD) {
+                jpart();
+                kpart();
+            } else {
+                kpart();
+                jpart();
+            }
+            lu_solve3<double>(c.f, r[0], r[1], r[2], o);
+        } else {
+            // f*(r - a*x - b*y) with the terms of the reference's branches (f*r == r*f)
+            double t = c.r[0];
+            auto jpart = [&]() {
+                const double u = keep(t - c.jn[0] * own[0]);
+                t = JSEL ? (jterm ? u : t) : u;
+            };
+            auto kpart = [&]() {
+                const double u = keep(t - c.kn[0] * nb[0]);
+                t = KSEL ? (kterm ? u : t) : u;
+            };
+            if constexpr (FWD) {
+                jpart();
+                kpart();
+            } else {
+                kpart();
+                jpart();
+            }
+            o[0] = c.f[0] * t;
+        }
+        st_vec<NEQ>(o, rO, vo, pstride);   // inactive positions are padding
+#pragma unroll
+        for (int m = 0; m < NEQ; ++m) edge_st(o[m], rEout, vout + m * 8, eout);
+#pragma unroll
+        for (int m = 0; m < NEQ; ++m) own[m] = o[m];
+        load(c, vp, ein_pf);
+        constexpr int ED = FWD ? 32 : -32;
+        h += FWD ? 1 : -1;
+        vo += dstep;
+        vp += dstep;
+        ein_pf += ED;
+        eout += ED;
+    }
+
+    template <bool JSEL, bool KSEL>
+    SMAC_DEV void chunks(int n)
+    {
+        for (int i = 0; i < n; ++i) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) step<JSEL, KSEL>(s[u]);
+        }
+    }
+};
+
+template <int NEQ, int PF, bool FWD>
+__global__ void __launch_bounds__(BAND_LANES)
+k_band(Dims d, LusgsWs w)
+{
+    constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ>();
+    const int lane = threadIdx.x, nband = gridDim.x;
+    // backward bands are numbered top down so that workgroups start in dependency order
+    const int b = FWD ? (int)blockIdx.x : nband - 1 - (int)blockIdx.x;
+    const int k0 = b * w.rows;
+    const int klast = (k0 + w.rows < d.km ? k0 + w.rows : d.km) - 1;
+    const bool real = lane < w.rows;
+    const int hbeg = k0, hend = d.jm - 1 + klast;
+    const int nsteps = hend - hbeg + 1;
+    const int dir = FWD ? 1 : -1;
+    const int h0 = FWD ? hbeg : hend;
+
+    Band<NEQ, PF, FWD> B;
+    B.d = d;
+    B.k = real ? k0 + lane : w.ks - 1;          // idle lanes run on a padding row
+    // hand-over rows: the consumer row takes its k-neighbour from the next band over
+    const bool has_in = FWD ? b > 0 : b + 1 < nband;
+    const bool has_out = FWD ? b + 1 < nband : b > 0;
+    B.kin = FWD ? k0 : klast;
+    B.kout = FWD ? klast : k0;
+    B.lane_in = has_in && real && B.k == B.kin;
+    B.vout = (has_out && real && B.k == B.kout) ? 0u : 0x80000000u;
+    // edge regions: forward sweep bands 0..nband-1, backward sweep nband..2*nband-1, then a
+    // region of zeros for bands with no neighbour on the consuming side
+    const unsigned long long *ebase = w.edge + (FWD ? 0 : (long)nband * w.epitch);
+    const unsigned long long *ein = has_in ? ebase + (long)(b - dir) * w.epitch
+                                           : w.edge + 2L * nband * w.epitch;
+    const unsigned ebytes = (unsigned)w.epitch * 8u;
+    B.rEin = mk_rsrc(ein, ebytes);
+    B.ein_ptr = ein;
+    B.rEout = mk_rsrc(ebase + (long)b * w.epitch, ebytes - 8);
+    B.stall = w.stall;
+    B.broken = false;
+    // the k-neighbour term is not formed on the first (FWD) / last grid row
+    B.kterm = FWD ? B.k > 0 : B.k < d.km - 1;
+
+    B.pstride = (unsigned)(w.ns * PW * 8);                                // bytes per plane pair
+    const unsigned gbytes = (unsigned)pairs_of<NEQ>(NB) * B.pstride;     // bytes per block group
+    const unsigned vbytes = (unsigned)pairs_of<NEQ>(NEQ) * B.pstride;
+    const char *lhs = (const char *)w.lhs;
+    // The last hyperplane of every plane is never touched (sweeps reach HP0 - 1 past
+    // their ends), so each range may stop a few bytes short of its group; distinct ranges
+    // keep the compiler from merging descriptor words and re-assembling descriptors per load.
+    B.rF = mk_rsrc(lhs, gbytes - 16);
+    B.rJ = mk_rsrc(lhs + (FWD ? 1 : 3) * (size_t)gbytes, gbytes - 32);
+    B.rK = mk_rsrc(lhs + (FWD ? 2 : 4) * (size_t)gbytes, gbytes - 48);
+    B.rR = mk_rsrc(w.r, vbytes - 64);
+    B.rO = FWD ? B.rR : mk_rsrc(w.dq, vbytes - 80);
+    B.dstep = dir * w.ks * PW * 8;
+    B.vo = (unsigned)(((long)(h0 + HP0) * w.ks + B.k) * PW * 8);   // current step
+    B.vp = B.vo + PF * B.dstep;                                     // prefetch step
+    B.h = h0;
+    B.eout = (unsigned)(h0 - B.kout + EPAD) * 32u;
+    const unsigned ein0 = (unsigned)(h0 - B.kin + EPAD) * 32u;
+    B.ein_pf = ein0 + PF * dir * 32;
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) B.own[m] = 0.0;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) B.load(B.s[u], B.vo + u * B.dstep, ein0 + u * dir * 32);
+
+    // the first klast-k0+1 steps are the only ones where a lane meets the j boundary
+    const int nch = (nsteps + PF - 1) / PF;
+    int nramp = (klast - k0 + 1 + PF - 1) / PF;
+    nramp = nramp < nch ? nramp : nch;
+    const bool krow = FWD ? k0 == 0 : klast == d.km - 1;
+    if (krow) {
+        B.template chunks<true, true>(nramp);
+        B.template chunks<false, true>(nch - nramp);
     } else {
-#pragma unroll
-        for (int m = 0; m < NEQ; ++m) s.e[m] = 0ull;
-    }
-}
-
-SMAC_DEV double shfl_prev(double v, int lane)   // value of lane-1
-{
-    return __shfl_up(v, 1, BAND_LANES);
-}
-
-SMAC_DEV double shfl_next(double v, int lane)   // value of lane+1
-{
-    return __shfl_down(v, 1, BAND_LANES);
-}
-
-// Forward sweep (ilusol3 / ilusol1 forward part, ref/ilusol.f:127-202 and 502-540): writes the
-// forward result into the workspace rhs planes.
-template <typename LT, int NEQ, int PF>
-__global__ void __launch_bounds__(BAND_LANES)
-k_fwd_band(Dims d, LusgsWs w)
-{
-    constexpr int NB = NEQ * NEQ;
-    using AT = LT;   // arithmetic type of the sweep
-    const int lane = threadIdx.x, b = blockIdx.x, nband = gridDim.x;
-    const int k0 = b * w.rows, k = k0 + lane;
-    const bool inband = lane < w.rows && k < d.km;
-    const int klast = (k0 + w.rows < d.km ? k0 + w.rows : d.km) - 1;
-    const int hbeg = k0, hend = d.jm - 1 + klast;
-    const LT *wl = (const LT *)w.lhs;
-    const LT *F = wl, *A = wl + (long)NB * w.ns, *B = wl + 2L * NB * w.ns;
-    double *R = w.r;
-    const bool consumer = b > 0 && lane == 0;
-    const bool producer = b + 1 < nband && k == klast;
-    const unsigned long long *ein = w.edge + (long)(b > 0 ? b - 1 : 0) * d.jm * 4;
-    unsigned long long *eout = w.edge + (long)b * d.jm * 4;
-    bool broken = false;
-
-    AT own[NEQ];
-#pragma unroll
-    for (int m = 0; m < NEQ; ++m) own[m] = (AT)0;
-
-    Slot<LT, NEQ> s[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u)
-        slot_load<LT, NEQ>(s[u], w, F, A, B, R, hbeg + u, k, consumer, ein, hbeg + u - k0, d.jm);
-
-    for (int h0 = hbeg; h0 <= hend; h0 += PF) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int h = h0 + u;
-            if (h <= hend) {
-                const int j = h - k;
-                const bool act = inband && j >= 0 && j < d.jm;
-                AT nb[NEQ];
-#pragma unroll
-                for (int m = 0; m < NEQ; ++m) nb[m] = (AT)shfl_prev((double)own[m], lane);
-                if (consumer && act) {
-                    edge_wait<NEQ>(ein + (long)j * 4, s[u].e, broken, w.stall);
-#pragma unroll
-                    for (int m = 0; m < NEQ; ++m) nb[m] = (AT)__longlong_as_double((long long)s[u].e[m]);
-                }
-                AT o[NEQ];
-                if constexpr (NEQ == 3) {
-                    AT r[3] = {(AT)s[u].r[0], (AT)s[u].r[1], (AT)s[u].r[2]};
-                    if (j > 0) {
-#pragma unroll
-                        for (int m = 0; m < 3; ++m)
-                            r[m] = r[m] - s[u].a[m] * own[0] - s[u].a[m + 3] * own[1] -
-                                   s[u].a[m + 6] * own[2];
-                    }
-                    if (k > 0) {
-                        // at j = 1 the reference's bjm(1,k)*rhs(jmax,k-1) is an exact zero
-#pragma unroll
-                        for (int m = 0; m < 3; ++m)
-                            r[m] = r[m] - s[u].b[m] * nb[0] - s[u].b[m + 3] * nb[1] -
-                                   s[u].b[m + 6] * nb[2];
-                    }
-                    lu_solve3<AT>(s[u].f, r[0], r[1], r[2], o);
-                } else {
-                    const AT rr = (AT)s[u].r[0], f = s[u].f[0];
-                    if (k == 0)
-                        o[0] = j == 0 ? rr * f : f * (rr - s[u].a[0] * own[0]);
-                    else if (j == 0)
-                        o[0] = f * (rr - s[u].b[0] * nb[0]);
-                    else
-                        o[0] = f * (rr - s[u].a[0] * own[0] - s[u].b[0] * nb[0]);
-                }
-                if (act) {
-                    const long i = (long)h * w.ks + k;
-#pragma unroll
-                    for (int m = 0; m < NEQ; ++m) R[m * w.ns + i] = (double)o[m];
-                    if (producer)
-#pragma unroll
-                        for (int m = 0; m < NEQ; ++m) edge_st(eout + (long)j * 4 + m, (double)o[m]);
-                }
-#pragma unroll
-                for (int m = 0; m < NEQ; ++m) own[m] = act ? o[m] : (AT)0;
-            }
-            slot_load<LT, NEQ>(s[u], w, F, A, B, R, h + PF, k, consumer, ein, h + PF - k0, d.jm);
-        }
-    }
-}
-
-// Backward sweep (ref/ilusol.f:353-469 and 560-591): reads the middle-step rhs, writes dq.
-template <typename LT, int NEQ, int PF>
-__global__ void __launch_bounds__(BAND_LANES)
-k_bwd_band(Dims d, LusgsWs w)
-{
-    constexpr int NB = NEQ * NEQ;
-    using AT = LT;
-    // bands are numbered top down so that workgroups are dispatched in dependency order
-    const int lane = threadIdx.x, nband = gridDim.x, b = nband - 1 - (int)blockIdx.x;
-    const int k0 = b * w.rows, k = k0 + lane;
-    const bool inband = lane < w.rows && k < d.km;
-    const int klast = (k0 + w.rows < d.km ? k0 + w.rows : d.km) - 1;
-    const int hbeg = k0, hend = d.jm - 1 + klast;
-    const LT *wl = (const LT *)w.lhs;
-    const LT *F = wl, *C = wl + 3L * NB * w.ns, *D = wl + 4L * NB * w.ns;
-    const double *R = w.r;
-    double *DQ = w.dq;
-    const bool consumer = b + 1 < nband && k == klast;
-    const bool producer = b > 0 && lane == 0;
-    unsigned long long *eb = w.edge + (long)nband * d.jm * 4;   // backward half
-    const unsigned long long *ein = eb + (long)(b + 1 < nband ? b + 1 : b) * d.jm * 4;
-    unsigned long long *eout = eb + (long)b * d.jm * 4;
-    bool broken = false;
-
-    AT own[NEQ];
-#pragma unroll
-    for (int m = 0; m < NEQ; ++m) own[m] = (AT)0;
-
-    Slot<LT, NEQ> s[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u)
-        slot_load<LT, NEQ>(s[u], w, F, D, C, R, hend - u, k, consumer, ein, hend - u - k, d.jm);
-
-    for (int h0 = hend; h0 >= hbeg; h0 -= PF) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int h = h0 - u;
-            if (h >= hbeg) {
-                const int j = h - k;
-                const bool act = inband && j >= 0 && j < d.jm;
-                AT nb[NEQ];
-#pragma unroll
-                for (int m = 0; m < NEQ; ++m) nb[m] = (AT)shfl_next((double)own[m], lane);
-                if (consumer && act) {
-                    edge_wait<NEQ>(ein + (long)j * 4, s[u].e, broken, w.stall);
-#pragma unroll
-                    for (int m = 0; m < NEQ; ++m) nb[m] = (AT)__longlong_as_double((long long)s[u].e[m]);
-                }
-                AT o[NEQ];
-                // slot fields: a = bkp (k-neighbour), b = bjp (j-neighbour)
-                if constexpr (NEQ == 3) {
-                    AT r[3] = {(AT)s[u].r[0], (AT)s[u].r[1], (AT)s[u].r[2]};
-                    if (k < d.km - 1) {
-#pragma unroll
-                        for (int m = 0; m < 3; ++m)
-                            r[m] = r[m] - s[u].a[m] * nb[0] - s[u].a[m + 3] * nb[1] -
-                                   s[u].a[m + 6] * nb[2];
-                    }
-                    if (j < d.jm - 1) {
-                        // at j = jmax the reference's bjp(jmax,k)*dq(1,k+1) is an exact zero
-#pragma unroll
-                        for (int m = 0; m < 3; ++m)
-                            r[m] = r[m] - s[u].b[m] * own[0] - s[u].b[m + 3] * own[1] -
-                                   s[u].b[m + 6] * own[2];
-                    }
-                    lu_solve3<AT>(s[u].f, r[0], r[1], r[2], o);
-                } else {
-                    const AT rr = (AT)s[u].r[0], f = s[u].f[0];
-                    if (k == d.km - 1)
-                        o[0] = j == d.jm - 1 ? rr * f : f * (rr - s[u].b[0] * own[0]);
-                    else if (j == d.jm - 1)
-                        o[0] = f * (rr - s[u].a[0] * nb[0]);
-                    else
-                        o[0] = f * (rr - s[u].a[0] * nb[0] - s[u].b[0] * own[0]);
-                }
-                if (act) {
-                    const long i = (long)h * w.ks + k;
-#pragma unroll
-                    for (int m = 0; m < NEQ; ++m) DQ[m * w.ns + i] = (double)o[m];
-                    if (producer)
-#pragma unroll
-                        for (int m = 0; m < NEQ; ++m) edge_st(eout + (long)j * 4 + m, (double)o[m]);
-                }
-#pragma unroll
-                for (int m = 0; m < NEQ; ++m) own[m] = act ? o[m] : (AT)0;
-            }
-            slot_load<LT, NEQ>(s[u], w, F, D, C, R, h - PF, k, consumer, ein, h - PF - k, d.jm);
-        }
+        B.template chunks<true, false>(nramp);
+        B.template chunks<false, false>(nch - nramp);
     }
 }
 
@@ -492,18 +597,34 @@ k_bwd_band(Dims d, LusgsWs w)
 
 void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &w)
 {
+    if (w.pf3 == 0) w.pf3 = 3;
+    if (w.pf1 == 0) w.pf1 = 8;
     w.rows = rows < 1 ? 1 : (rows > BAND_LANES ? BAND_LANES : rows);
     w.nband = (km + w.rows - 1) / w.rows;
-    const int need = (w.nband - 1) * w.rows + BAND_LANES;   // every lane's row index fits
+    // every lane's row fits, and row ks-1 is padding (idle lanes work there)
+    int need = (w.nband - 1) * w.rows + BAND_LANES;
+    if (need < km + 1) need = km + 1;
     w.ks = (need + 63) / 64 * 64;
     w.H = jm + km - 2;
-    w.ns = (long)(w.H + 1) * w.ks;
-    w.nedge = 2L * w.nband * jm * 4;
+    w.ns = (long)(w.H + 1 + 2 * HP0) * w.ks;
+    w.epitch = (jm + 2 * EPAD) * 4;
+    w.nedge = (2L * w.nband + 1) * w.epitch;
 }
 
-long lusgs_ws_lhs_bytes(const LusgsWs &w, int neqs, int lt_bytes)
+long lusgs_ws_lhs_bytes(const LusgsWs &w, int neqs)
 {
-    return 5L * neqs * neqs * w.ns * lt_bytes;
+    return neqs == 3 ? 5L * pairs_of<3>(9) * w.ns * 2 * 8 : 5L * w.ns * 8;
+}
+
+long lusgs_ws_vec_bytes(const LusgsWs &w, int neqs)
+{
+    return neqs == 3 ? (long)pairs_of<3>(3) * w.ns * 2 * 8 : w.ns * 8;
+}
+
+bool lusgs_ws_fits(const LusgsWs &w)
+{
+    // buffer offsets are 32-bit: one block group (5 plane pairs) must stay below 2 GiB
+    return (long)pairs_of<3>(9) * w.ns * 2 * 8 < (1L << 31);
 }
 
 template <typename LT, int NEQ>
@@ -514,12 +635,27 @@ static void launch_lusgs(hipStream_t sm, const Dims &d, const LusgsWs &w, const 
     dim3 tg((d.jm + TJ - 1) / TJ, (d.km + TK - 1) / TK);
     hipLaunchKernelGGL((k_shear<LT, NEQ>), tg, dim3(TILE_THREADS), 0, sm, d, w, rhs, btc, bjm,
                        bkm, bjp, bkp, fac);
-    constexpr int PF = NEQ == 3 ? 3 : 6;
-    hipLaunchKernelGGL((k_fwd_band<LT, NEQ, PF>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w);
+    const int pf = NEQ == 3 ? w.pf3 : w.pf1;
+#define BAND_LAUNCH(PFV, FWDV)                                                                 \
+    hipLaunchKernelGGL((k_band<NEQ, PFV, FWDV>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
+#define BAND_SWITCH(FWDV)                                                                      \
+    do {                                                                                       \
+        switch (pf) {                                                                          \
+        case 1: BAND_LAUNCH(1, FWDV); break;                                                   \
+        case 2: BAND_LAUNCH(2, FWDV); break;                                                   \
+        case 4: BAND_LAUNCH(4, FWDV); break;                                                   \
+        case 6: BAND_LAUNCH(6, FWDV); break;                                                   \
+        case 8: BAND_LAUNCH(8, FWDV); break;                                                   \
+        default: BAND_LAUNCH(3, FWDV); break;                                                  \
+        }                                                                                      \
+    } while (0)
+    BAND_SWITCH(true);
     long nb = (w.ns + 255) / 256;
     hipLaunchKernelGGL((k_mid<LT, NEQ>), dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(256), 0, sm,
                        d, w);
-    hipLaunchKernelGGL((k_bwd_band<LT, NEQ, PF>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w);
+    BAND_SWITCH(false);
+#undef BAND_SWITCH
+#undef BAND_LAUNCH
     hipLaunchKernelGGL((k_unshear<NEQ>), tg, dim3(TILE_THREADS), 0, sm, d, w, w.dq, dq);
     if (rhs_out)
         hipLaunchKernelGGL((k_unshear<NEQ>), tg, dim3(TILE_THREADS), 0, sm, d, w, w.r, rhs);

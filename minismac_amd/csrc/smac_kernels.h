@@ -55,22 +55,27 @@ void launch_wake(hipStream_t, const Dims &, int neqs, const WakePairs &, const d
                  double *ds, double *rhs);
 
 // lusgs.hip: LU-SGS with hyperplane (j+k = const) sweeps (ref/lusgs.f, ref/ilusol.f).
-// Workspace in the sheared layout: point (j,k) at (j+k)*ks + k of every plane.
+// Workspace in the sheared layout: point (j,k) at (j+k+16)*ks + k of every plane, planes
+// stored in pairs for the 3x3 system.
 struct LusgsWs {
     int rows;                    // rows per band (one wavefront each), 1..64
     int nband;                   // bands covering k = 0..km-1
     int ks;                      // pitch of one hyperplane
     int H;                       // last hyperplane, jm + km - 2
-    long ns;                     // (H+1)*ks, plane stride
-    long nedge;                  // 2*nband*jm*4 slots
+    long ns;                     // positions per plane, (H+33)*ks
+    int epitch;                  // slots per band region: (jm + 2*128 padding)*4
+    long nedge;                  // (2*nband + 1)*epitch slots
     void *lhs;                   // 5 block groups (factors, bjm, bkm, bjp, bkp), LT
     double *r;                   // neqs planes: rhs, forward result, middle result
     double *dq;                  // neqs planes
     unsigned long long *edge;    // band hand-over slots (forward half, backward half)
     int *stall;                  // raised when a hand-over wait times out
+    int pf3, pf1;                // prefetch distance (steps) of the 3x3 and scalar sweeps
 };
 void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &);
-long lusgs_ws_lhs_bytes(const LusgsWs &, int neqs, int lt_bytes);
+long lusgs_ws_lhs_bytes(const LusgsWs &, int neqs);   // the five block groups
+long lusgs_ws_vec_bytes(const LusgsWs &, int neqs);   // one vector (rhs or dq)
+bool lusgs_ws_fits(const LusgsWs &);                  // 32-bit buffer offsets suffice
 // fac (nullable) receives the diagonal factors in the natural layout (the reference's scr1);
 // rhs_out also returns the middle-step rhs in the natural layout, as the reference leaves it
 template <typename LT>
