@@ -35,6 +35,11 @@ struct smac_handle {
     Dims d;
     int device = 0;
     hipStream_t stream = nullptr;
+    // The SA model does not feed the mean flow within a pass (the viscous terms use the
+    // laminar viscosity, ref/visc.f:97), so it runs on a second stream beside it.
+    hipStream_t stream_sa = nullptr;
+    hipEvent_t ev_q = nullptr, ev_sa_read = nullptr, ev_sa_done = nullptr;
+    LusgsWs lws_sa = {};       // the SA's own LU-SGS workspace (neqs = 1)
     // grid (device)
     double *x = nullptr, *y = nullptr, *met = nullptr, *dj = nullptr;
     // flow state and LHS (device, FP64)
@@ -98,25 +103,25 @@ static void free_lusgs_ws(smac_handle *h)
     for (void *p : h->lws_allocs) (void)hipFree(p);
     h->lws_allocs.clear();
     h->lws = LusgsWs{};
+    h->lws_sa = LusgsWs{};
 }
 
-// LU-SGS workspace for the mean flow (neqs = 3) and, in the same planes, the SA (neqs = 1)
-static smac_status alloc_lusgs_ws(smac_handle *h, int rows)
+static smac_status alloc_one_ws(smac_handle *h, int rows, int neqs, LusgsWs &w)
 {
-    const int pf3 = h->lws.pf3, pf1 = h->lws.pf1;
-    free_lusgs_ws(h);
-    LusgsWs &w = h->lws;
+    const int pf3 = w.pf3, pf1 = w.pf1;
+    w = LusgsWs{};
     w.pf3 = pf3;
     w.pf1 = pf1;
     lusgs_ws_shape(h->d.jm, h->d.km, rows, w);
     if (!lusgs_ws_fits(w)) return fail(SMAC_ERR_SIZE, "grid too large for the LU-SGS workspace");
     void *p[4] = {};
-    size_t sz[4] = {(size_t)lusgs_ws_lhs_bytes(w, 3), (size_t)lusgs_ws_vec_bytes(w, 3),
-                    (size_t)lusgs_ws_vec_bytes(w, 3), (size_t)(w.nedge * 8)};
+    size_t sz[4] = {(size_t)lusgs_ws_lhs_bytes(w, neqs), (size_t)lusgs_ws_vec_bytes(w, neqs),
+                    (size_t)lusgs_ws_vec_bytes(w, neqs), (size_t)(w.nedge * 8)};
     for (int i = 0; i < 4; ++i) {
         HIP_TRY(hipMalloc(&p[i], sz[i] + 256));
         h->lws_allocs.push_back(p[i]);
-        // the padding of the sheared planes is read (and ignored) by idle lanes: keep it finite
+        // the padding of the sheared planes is read (and ignored) by idle lanes, and the
+        // hand-over padding must not hold the sentinel: start from zeros
         HIP_TRY(hipMemset(p[i], 0, sz[i] + 256));
     }
     w.lhs = p[0];
@@ -124,7 +129,23 @@ static smac_status alloc_lusgs_ws(smac_handle *h, int rows)
     w.dq = (double *)p[2];
     w.edge = (unsigned long long *)p[3];
     w.stall = &h->st->stall;
-    h->lusgs_rows = w.rows;
+    return SMAC_OK;
+}
+
+// LU-SGS workspaces: the mean flow (neqs = 3) and the SA (neqs = 1), which may run at once
+static smac_status alloc_lusgs_ws(smac_handle *h, int rows)
+{
+    const LusgsWs keep = h->lws, keep_sa = h->lws_sa;
+    free_lusgs_ws(h);
+    h->lws.pf3 = keep.pf3;
+    h->lws.pf1 = keep.pf1;
+    h->lws_sa.pf3 = keep_sa.pf3;
+    h->lws_sa.pf1 = keep_sa.pf1;
+    smac_status rc = alloc_one_ws(h, rows, 3, h->lws);
+    if (rc) return rc;
+    rc = alloc_one_ws(h, rows, 1, h->lws_sa);
+    if (rc) return rc;
+    h->lusgs_rows = h->lws.rows;
     return SMAC_OK;
 }
 
@@ -136,6 +157,9 @@ static void free_handle(smac_handle *h)
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream_sa) (void)hipStreamDestroy(h->stream_sa);
+    for (hipEvent_t e : {h->ev_q, h->ev_sa_read, h->ev_sa_done})
+        if (e) (void)hipEventDestroy(e);
     delete h;
 }
 
@@ -246,7 +270,13 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
         }                          \
     } while (0)
     (void)hipGetDevice(&h->device);
-    if (hipStreamCreate(&h->stream) != hipSuccess) { free_handle(h); return fail(SMAC_ERR_HIP, "hipStreamCreate failed"); }
+    if (hipStreamCreate(&h->stream) != hipSuccess || hipStreamCreate(&h->stream_sa) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_q, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_sa_read, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_sa_done, hipEventDisableTiming) != hipSuccess) {
+        free_handle(h);
+        return fail(SMAC_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
+    }
     TRY(dalloc(h, &h->x, N));
     TRY(dalloc(h, &h->y, N));
     TRY(dalloc(h, &h->met, 4 * N));
@@ -306,6 +336,7 @@ smac_status smac_destroy(smac_handle *h)
 {
     if (!h) return fail(SMAC_ERR_ARG, "handle is NULL");
     (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->stream_sa);
     free_handle(h);
     return SMAC_OK;
 }
@@ -331,7 +362,8 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
         const bool ok = value == 1 || value == 2 || value == 3 || value == 4 || value == 6 ||
                         value == 8;
         if (!ok) return fail(SMAC_ERR_ARG, std::string(name) + ": unsupported prefetch distance");
-        (mean ? h->lws.pf3 : h->lws.pf1) = value;
+        if (mean) h->lws.pf3 = value;
+        else h->lws_sa.pf1 = value;
         return SMAC_OK;
     }
     if (!std::strcmp(name, "lusgs_rows")) {
@@ -387,12 +419,22 @@ static smac_status enqueue_pass(smac_handle *h)
            *bkp = h->blk[4], *scr1 = h->blk[5];
     h->nt += 1;
     mark(h, 0);
-    if (dp.ivis == 1 && dp.iturb == 1) {
-        if (h->first_sa) launch_sa_first(sm, d, h->walls, h->x, h->y, h->sa);
-        launch_sa_assemble(sm, d, dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
+    const bool sa = dp.ivis == 1 && dp.iturb == 1;
+    // timing mode keeps every family on one stream so that the events bracket them
+    const bool side = sa && !h->timing;
+    hipStream_t ss = side ? h->stream_sa : sm;
+    if (sa) {
+        if (side) {
+            HIP_TRY(hipEventRecord(h->ev_q, sm));             // q of the previous pass
+            HIP_TRY(hipStreamWaitEvent(ss, h->ev_q, 0));
+        }
+        if (h->first_sa) launch_sa_first(ss, d, h->walls, h->x, h->y, h->sa);
+        launch_sa_assemble(ss, d, dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
                            (int)h->regs.size(), h->regs.data(), h->sa);
-        launch_sa_solve(sm, d, dp, h->p.ntjsp, h->wake, &h->lws, h->sa);
-        launch_sa_update(sm, d, dp, h->w, h->st, h->sa);
+        if (side) HIP_TRY(hipEventRecord(h->ev_sa_read, ss));  // done reading q
+        launch_sa_solve(ss, d, dp, h->p.ntjsp, h->wake, &h->lws_sa, h->sa);
+        launch_sa_update(ss, d, dp, h->w, h->st, h->sa);
+        if (side) HIP_TRY(hipEventRecord(h->ev_sa_done, ss));
         h->first_sa = false;
     }
     mark(h, 1);
@@ -414,7 +456,9 @@ static smac_status enqueue_pass(smac_handle *h)
                               bkp, nullptr, false);
     (void)scr1;
     mark(h, 4);
+    if (side) HIP_TRY(hipStreamWaitEvent(sm, h->ev_sa_read, 0));   // q is overwritten next
     launch_update_force(sm, d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq, h->w, h->st);
+    if (side) HIP_TRY(hipStreamWaitEvent(sm, h->ev_sa_done, 0));   // the pass ends with both
     mark(h, 5);
     HIP_TRY(hipGetLastError());
     return SMAC_OK;
