@@ -125,8 +125,8 @@ static smac_status alloc_one_ws(smac_handle *h, int rows, int neqs, LusgsWs &w)
         HIP_TRY(hipMemset(p[i], 0, sz[i] + 256));
     }
     w.lhs = p[0];
-    w.r = (double *)p[1];
-    w.dq = (double *)p[2];
+    w.r = p[1];
+    w.dq = p[2];
     w.edge = (unsigned long long *)p[3];
     w.stall = &h->st->stall;
     return SMAC_OK;
@@ -177,8 +177,8 @@ static smac_status check_params(const smac_params *p)
     if (p->njsp < 0 || p->nksp < 0 || p->ntjsp < 0) return fail(SMAC_ERR_ARG, "negative sweep count");
     if (p->ntksp != 0)
         return fail(SMAC_ERR_ARG, "ntksp > 0 (SA eta sweeps) is not supported; the reference uses 0");
-    if (p->precision != SMAC_FP64)
-        return fail(SMAC_ERR_ARG, "only SMAC_FP64 is available in this build");
+    if (p->precision != SMAC_FP64 && p->precision != SMAC_FP32_LHS)
+        return fail(SMAC_ERR_ARG, "precision must be SMAC_FP64 or SMAC_FP32_LHS");
     if (p->line_order != SMAC_ORDER_REFERENCE)
         return fail(SMAC_ERR_ARG, "only SMAC_ORDER_REFERENCE is available in this build");
     if (!(p->dtau > 0.0) || !(p->reynum > 0.0) || !(p->beta > 0.0) || !(p->dt > 0.0))
@@ -200,6 +200,8 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
     *out = nullptr;
     smac_status rc = check_params(p);
     if (rc) return rc;
+    if (p->precision == SMAC_FP32_LHS && p->impsch != 2)
+        return fail(SMAC_ERR_ARG, "SMAC_FP32_LHS is available with LU-SGS (impsch = 2)");
     if (!x || !y || !rtxy || !dj || !regs || nreg < 1)
         return fail(SMAC_ERR_ARG, "grid, metric or BC arrays missing");
     if (jmax < 5 || kmax < 5 || jmax > 8100 || kmax > 8100)
@@ -345,6 +347,8 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
 {
     if (!h || !name) return fail(SMAC_ERR_ARG, "handle or name is NULL");
     if (!std::strcmp(name, "keep_lhs")) {
+        if (value && h->p.precision != SMAC_FP64)
+            return fail(SMAC_ERR_ARG, "keep_lhs keeps FP64 blocks; the handle is FP32-LHS");
         h->keep_lhs = value != 0;
         if (h->keep_lhs && !h->lhs_copy[0])
             for (int b = 0; b < 5; ++b) {
@@ -438,10 +442,21 @@ static smac_status enqueue_pass(smac_handle *h)
         h->first_sa = false;
     }
     mark(h, 1);
-    launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm, bkp);
-    for (const BcRegion &r : h->regs)
-        launch_bc<double>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm, bjp,
-                          bkm, bkp);
+    const bool f32 = h->p.precision == SMAC_FP32_LHS;
+    if (f32) {
+        // FP32-LHS: blocks stored (and factored, solved) in FP32 in the same allocations
+        float *fb[5] = {(float *)btc, (float *)bjm, (float *)bjp, (float *)bkm, (float *)bkp};
+        launch_assemble<float>(sm, d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2], fb[3],
+                               fb[4]);
+        for (const BcRegion &r : h->regs)
+            launch_bc<float>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, fb[0], fb[1],
+                             fb[2], fb[3], fb[4]);
+    } else {
+        launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm, bkp);
+        for (const BcRegion &r : h->regs)
+            launch_bc<double>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm, bjp,
+                              bkm, bkp);
+    }
     mark(h, 2);
     launch_resid(sm, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
     if (h->keep_lhs)
@@ -451,6 +466,10 @@ static smac_status enqueue_pass(smac_handle *h)
     if (h->p.impsch == 1)
         launch_lrelax_ref<double>(sm, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
                                   h->dq, h->s, btc, bjm, bjp, bkm, bkp);
+    else if (f32)
+        launch_lusgs3<float>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, (const float *)btc,
+                             (const float *)bjm, (const float *)bjp, (const float *)bkm,
+                             (const float *)bkp, nullptr, false);
     else
         launch_lusgs3<double>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm,
                               bkp, nullptr, false);
@@ -619,6 +638,8 @@ smac_status smac_lrelax(const smac_params *p, int jmax, int kmax, int npair, con
     g_err.clear();
     smac_status rc = check_params(p);
     if (rc) return rc;
+    if (p->precision != SMAC_FP64)
+        return fail(SMAC_ERR_ARG, "the host-array drop-ins solve the reference's FP64 system");
     if (!q || !dq || !s || !btc || !bjm || !bjp || !bkm || !bkp)
         return fail(SMAC_ERR_ARG, "array argument is NULL");
     if (jmax < 3 || kmax < 3 || jmax > 2700 || kmax > 2700)
@@ -658,6 +679,8 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
 {
     g_err.clear();
     if (!p) return fail(SMAC_ERR_ARG, "params is NULL");
+    if (p->precision != SMAC_FP64)
+        return fail(SMAC_ERR_ARG, "the host-array drop-ins solve the reference's FP64 system");
     if (neqs != 1 && neqs != 3) return fail(SMAC_ERR_NEQS, "neqs must be 1 or 3 (ref/lusgs.f:108-115)");
     if (!q || !dq || !rhs || !btc || !bjm || !bjp || !bkm || !bkp || !scr1)
         return fail(SMAC_ERR_ARG, "array argument is NULL");
@@ -690,8 +713,8 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
     if (!lusgs_ws_fits(ws)) return fail(SMAC_ERR_SIZE, "grid too large for the LU-SGS workspace");
     const long lb = lusgs_ws_lhs_bytes(ws, neqs), vb = lusgs_ws_vec_bytes(ws, neqs);
     ws.lhs = b.get<char>(lb);
-    ws.r = (double *)b.get<char>(vb);
-    ws.dq = (double *)b.get<char>(vb);
+    ws.r = b.get<char>(vb);
+    ws.dq = b.get<char>(vb);
     ws.edge = b.get<unsigned long long>(ws.nedge);
     ws.stall = b.get<int>(1);
     if (!ws.lhs || !ws.r || !ws.dq || !ws.edge || !ws.stall) return fail(SMAC_ERR_HIP, "hipMalloc failed");

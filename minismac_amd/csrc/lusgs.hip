@@ -45,13 +45,17 @@ constexpr int EPAD = 128;        // padding slots on either side of a band's edg
 // position of point (j,k) in the sheared planes
 SMAC_DEV long SX(const LusgsWs &w, int j, int k) { return (long)(j + k + HP0) * w.ks + k; }
 
-// planes per pair (2 for the 3x3 system, 1 for the scalar one) and element index of plane m
-template <int NEQ> constexpr int pair_width() { return NEQ == 3 ? 2 : 1; }
-template <int NEQ> constexpr int pairs_of(int planes) { return (planes + pair_width<NEQ>() - 1) / pair_width<NEQ>(); }
-template <int NEQ>
+// Planes per 16-byte group for the 3x3 system (2 doubles or 4 floats: one load fetches a
+// group), 1 for the scalar one; element index of plane m at position pos.
+template <int NEQ, typename T> constexpr int pair_width() { return NEQ == 3 ? 16 / (int)sizeof(T) : 1; }
+template <int NEQ, typename T> constexpr int pairs_of(int planes)
+{
+    return (planes + pair_width<NEQ, T>() - 1) / pair_width<NEQ, T>();
+}
+template <int NEQ, typename T>
 SMAC_DEV long EI(long ns, int m, long pos)
 {
-    constexpr int PW = pair_width<NEQ>();
+    constexpr int PW = pair_width<NEQ, T>();
     return ((long)(m / PW) * ns + pos) * PW + (m % PW);
 }
 
@@ -100,7 +104,7 @@ SMAC_DEV void tile_to_sheared(const LusgsWs &w, int jm, int km, int j0, int k0,
         if (jj < 0 || jj >= TJ || j >= jm || k >= km) continue;
         const long o = SX(w, j, k);
 #pragma unroll
-        for (int m = 0; m < NP; ++m) dst[EI<NEQ>(w.ns, m, o)] = (LT)lds[m][kk][jj];
+        for (int m = 0; m < NP; ++m) dst[EI<NEQ, LT>(w.ns, m, o)] = (LT)lds[m][kk][jj];
     }
 }
 
@@ -126,11 +130,11 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         const LT *__restrict__ bjm, const LT *__restrict__ bkm, const LT *__restrict__ bjp,
         const LT *__restrict__ bkp, LT *__restrict__ fac)
 {
-    constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ>();
+    constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ, LT>();
     __shared__ double lds[NB][TK][TPAD];
     const int j0 = blockIdx.x * TJ, k0 = blockIdx.y * TK;
     LT *wl = (LT *)w.lhs;
-    const long gstride = (long)pairs_of<NEQ>(NB) * w.ns * PW;   // elements per block group
+    const long gstride = (long)pairs_of<NEQ, LT>(NB) * w.ns * PW;   // elements per block group
 
     {
         // hand-over slots of both sweeps: the jm real ones become the sentinel, the padding
@@ -177,13 +181,13 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
     __syncthreads();
     tile_from_natural<double, NEQ>(d, j0, k0, lds, rhs);
     __syncthreads();
-    tile_to_sheared<double, NEQ, NEQ>(w, d.jm, d.km, j0, k0, lds, w.r);
+    tile_to_sheared<LT, NEQ, NEQ>(w, d.jm, d.km, j0, k0, lds, (LT *)w.r);   // the solve's type
 }
 
-// sheared -> natural for the NEQ planes of a vector (dq, or the middle-step rhs)
-template <int NEQ>
+// sheared -> natural (FP64) for the NEQ planes of a vector (dq, or the middle-step rhs)
+template <int NEQ, typename T>
 __global__ void __launch_bounds__(TILE_THREADS)
-k_unshear(Dims d, LusgsWs w, const double *__restrict__ src, double *__restrict__ dst)
+k_unshear(Dims d, LusgsWs w, const T *__restrict__ src, double *__restrict__ dst)
 {
     constexpr int NP = NEQ;
     __shared__ double lds[NP][TK][TPAD];
@@ -194,7 +198,7 @@ k_unshear(Dims d, LusgsWs w, const double *__restrict__ src, double *__restrict_
         if (jj < 0 || jj >= TJ || j >= d.jm || k >= d.km) continue;
         const long o = SX(w, j, k);
 #pragma unroll
-        for (int m = 0; m < NP; ++m) lds[m][kk][jj] = src[EI<NEQ>(w.ns, m, o)];
+        for (int m = 0; m < NP; ++m) lds[m][kk][jj] = (double)src[EI<NEQ, T>(w.ns, m, o)];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < TK * TJ; e += TILE_THREADS) {
@@ -215,31 +219,33 @@ __global__ void k_mid(Dims d, LusgsWs w)
 {
     constexpr int NB = NEQ * NEQ;
     const LT *F = (const LT *)w.lhs;
-    double *R = w.r;
+    LT *R = (LT *)w.r;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < w.ns;
          i += (long)gridDim.x * blockDim.x) {
         const int h = (int)(i / w.ks) - HP0, k = (int)(i % w.ks), j = h - k;
         if (k >= d.km || j < 0 || j >= d.jm) continue;
         if constexpr (NEQ == 3) {
-            double f[NB];
+            LT f[NB];
 #pragma unroll
-            for (int m = 0; m < NB; ++m) f[m] = (double)F[EI<NEQ>(w.ns, m, i)];
-            const long i0 = EI<NEQ>(w.ns, 0, i), i1 = EI<NEQ>(w.ns, 1, i), i2 = EI<NEQ>(w.ns, 2, i);
-            const double o0 = R[i0], o1 = R[i1], o2 = R[i2];
-            double b11 = 1. / f[0];
-            double b12 = f[3] * b11;
-            double b13 = f[6] * b11;
-            double b21 = f[1];
-            double b22 = 1. / f[4] + f[1] * f[3];
-            double b23 = f[7] / f[4] + f[1] * f[6];
-            double b31 = f[2];
-            double b32 = f[5] + f[2] * f[3];
-            double b33 = 1. / f[8] + f[2] * f[6] + f[5] * f[7];
+            for (int m = 0; m < NB; ++m) f[m] = F[EI<NEQ, LT>(w.ns, m, i)];
+            const long i0 = EI<NEQ, LT>(w.ns, 0, i), i1 = EI<NEQ, LT>(w.ns, 1, i),
+                       i2 = EI<NEQ, LT>(w.ns, 2, i);
+            const LT o0 = R[i0], o1 = R[i1], o2 = R[i2];
+            const LT one = (LT)1.;
+            LT b11 = one / f[0];
+            LT b12 = f[3] * b11;
+            LT b13 = f[6] * b11;
+            LT b21 = f[1];
+            LT b22 = one / f[4] + f[1] * f[3];
+            LT b23 = f[7] / f[4] + f[1] * f[6];
+            LT b31 = f[2];
+            LT b32 = f[5] + f[2] * f[3];
+            LT b33 = one / f[8] + f[2] * f[6] + f[5] * f[7];
             R[i0] = b11 * o0 + b12 * o1 + b13 * o2;
             R[i1] = b21 * o0 + b22 * o1 + b23 * o2;
             R[i2] = b31 * o0 + b32 * o1 + b33 * o2;
         } else {
-            R[i] = R[i] / (double)F[i];
+            R[i] = R[i] / F[i];
         }
     }
 }
@@ -277,12 +283,49 @@ SMAC_DEV void bst1(double x, rsrc_t r, unsigned vo, unsigned so)
 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, x), r, (int)vo, (int)so, 0);
 }
+struct F4 {
+    float x, y, z, w;
+};
+SMAC_DEV F4 bld4f(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(F4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0));
+}
+SMAC_DEV float bld1f(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+}
+SMAC_DEV void bst4f(F4 v, rsrc_t r, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, v), r, (int)vo, (int)so, 0);
+}
 
-// NP planes of one group at the current position (vo), pair p at soffset p*pstride
+// NP planes of one group at the current position (vo), group p of planes at soffset p*pstride
+template <int NEQ, int NP>
+SMAC_DEV void ld_group(float *out, rsrc_t r, unsigned vo, unsigned pstride)
+{
+    static_assert(NEQ == 3, "the scalar sweep is FP64");
+#pragma unroll
+    for (int p = 0; p < NP / 4; ++p) {
+        F4 t = bld4f(r, vo, p * pstride);
+        out[4 * p] = t.x;
+        out[4 * p + 1] = t.y;
+        out[4 * p + 2] = t.z;
+        out[4 * p + 3] = t.w;
+    }
+    if constexpr (NP % 4 == 1) {
+        out[NP - 1] = bld1f(r, vo, (NP / 4) * pstride);
+    } else if constexpr (NP % 4 > 1) {
+        F4 t = bld4f(r, vo, (NP / 4) * pstride);   // the group's unused lanes are padding
+        const float v[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int m = 0; m < NP % 4; ++m) out[(NP / 4) * 4 + m] = v[m];
+    }
+}
+
 template <int NEQ, int NP>
 SMAC_DEV void ld_group(double *out, rsrc_t r, unsigned vo, unsigned pstride)
 {
-    if constexpr (pair_width<NEQ>() == 2) {
+    if constexpr (pair_width<NEQ, double>() == 2) {
 #pragma unroll
         for (int p = 0; p < NP / 2; ++p) {
             D2 t = bld2(r, vo, p * pstride);
@@ -294,6 +337,13 @@ SMAC_DEV void ld_group(double *out, rsrc_t r, unsigned vo, unsigned pstride)
 #pragma unroll
         for (int p = 0; p < NP; ++p) out[p] = bld1(r, vo, p * pstride);
     }
+}
+
+template <int NEQ>
+SMAC_DEV void st_vec(const float *v, rsrc_t r, unsigned vo, unsigned)
+{
+    static_assert(NEQ == 3, "the scalar sweep is FP64");
+    bst4f(F4{v[0], v[1], v[2], 0.f}, r, vo, 0);
 }
 
 template <int NEQ>
@@ -363,6 +413,14 @@ SMAC_DEV void edge_wait(const unsigned long long *p, unsigned long long e[NEQ], 
 
 // lane l receives lane l-1's value (FWD) or lane l+1's (backward); the end lane receives 0
 template <bool FWD>
+SMAC_DEV float wave_shift(float v)
+{
+    constexpr int ctrl = FWD ? 0x138 : 0x130;   // DPP wave_shr:1 / wave_shl:1, bound_ctrl
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl,
+                                                              0xf, 0xf, true));
+}
+
+template <bool FWD>
 SMAC_DEV double wave_shift(double v)
 {
     constexpr int ctrl = FWD ? 0x138 : 0x130;   // DPP wave_shr:1 / wave_shl:1, bound_ctrl
@@ -375,16 +433,28 @@ SMAC_DEV double wave_shift(double v)
 
 // Forces x to be computed where it stands: a term that is formed and then selected stays a
 // select instead of being sunk into a divergent branch around its computation.
-SMAC_DEV double keep(double x)
+template <typename T>
+SMAC_DEV T keep(T x)
 {
     asm volatile("" : "+v"(x));
     return x;
 }
 
+// FP32-LHS form of the 3x3 substitution (fused multiply-adds; not the reference's rounding)
+SMAC_DEV void lu_solve3_fma(const float f[9], float r1, float r2, float r3, float o[3])
+{
+    const float z1 = r1 * f[0];
+    const float z2 = __builtin_fmaf(-f[1], z1, r2) * f[4];
+    const float z3 = __builtin_fmaf(-f[5], z2, __builtin_fmaf(-f[2], z1, r3)) * f[8];
+    o[2] = z3;
+    o[1] = __builtin_fmaf(-f[7], o[2], z2);
+    o[0] = __builtin_fmaf(-f[3], o[1], __builtin_fmaf(-f[6], o[2], z1));
+}
+
 // Per-step operands of one lane, loaded PF steps ahead.
-template <int NEQ>
+template <int NEQ, typename T>
 struct Slot {
-    double f[NEQ * NEQ], jn[NEQ * NEQ], kn[NEQ * NEQ], r[NEQ];
+    T f[NEQ * NEQ], jn[NEQ * NEQ], kn[NEQ * NEQ], r[NEQ];
     unsigned long long e[NEQ];
 };
 
@@ -392,7 +462,8 @@ struct Slot {
 // (ref/ilusol.f:127-202 and 502-540) from the rhs planes into the rhs planes.  Backward:
 // ref/ilusol.f:353-469 and 560-591, from the middle-step rhs into the dq planes.
 // j-neighbour block: bjm (FWD) / bjp; k-neighbour block: bkm (FWD) / bkp.
-template <int NEQ, int PF, bool FWD>
+// T = double: the reference's arithmetic.  T = float (3x3 only): the FP32-LHS mode.
+template <int NEQ, int PF, bool FWD, typename T>
 struct Band {
     static constexpr int NB = NEQ * NEQ;
     Dims d;
@@ -405,13 +476,13 @@ struct Band {
     unsigned pstride;
     int dstep;
     // running state
-    double own[NEQ];
-    Slot<NEQ> s[PF];
+    T own[NEQ];
+    Slot<NEQ, T> s[PF];
     int h;
     unsigned vo, vp;       // byte offset of this lane's position: current / prefetch step
     unsigned ein_pf, eout; // soffset of the hand-over slots: prefetch step / current step
 
-    SMAC_DEV void load(Slot<NEQ> &sl, unsigned v, unsigned so)
+    SMAC_DEV void load(Slot<NEQ, T> &sl, unsigned v, unsigned so)
     {
         ld_group<NEQ, NB>(sl.f, rF, v, pstride);
         ld_group<NEQ, NB>(sl.jn, rJ, v, pstride);
@@ -427,9 +498,9 @@ struct Band {
     // form the j-neighbour term; KSEL: the band holds the first (FWD) / last grid row, where it
     // does not form the k-neighbour term.  Elsewhere both terms are formed unconditionally.
     template <bool JSEL, bool KSEL>
-    SMAC_DEV void step(Slot<NEQ> &c)
+    SMAC_DEV void step(Slot<NEQ, T> &c)
     {
-        double nb[NEQ];
+        T nb[NEQ];
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) nb[m] = wave_shift<FWD>(own[m]);
         bool empty = false;
@@ -439,13 +510,41 @@ struct Band {
             edge_wait<NEQ>(ein_ptr + (long)(h - kin + EPAD) * 4, c.e, broken, stall);
 #pragma unroll
         for (int m = 0; m < NEQ; ++m)
-            nb[m] = lane_in ? __longlong_as_double((long long)c.e[m]) : nb[m];
+            nb[m] = lane_in ? (T)__longlong_as_double((long long)c.e[m]) : nb[m];
         const int j = h - k;
         // FWD: no j-term at j = 1 (bjm(1,k)*rhs(jmax,k-1) is an exact zero); backward: none at
         // j = jmax (bjp(jmax,k)*dq(1,k+1), likewise)
         const bool jterm = FWD ? j > 0 : j < d.jm - 1;
-        double o[NEQ];
-        if constexpr (NEQ == 3) {
+        T o[NEQ];
+        if constexpr (NEQ == 3 && sizeof(T) == 4) {
+            T r[3] = {c.r[0], c.r[1], c.r[2]};
+            auto jpart = [&]() {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const T t = __builtin_fmaf(-c.jn[m + 6], own[2],
+                                __builtin_fmaf(-c.jn[m + 3], own[1],
+                                __builtin_fmaf(-c.jn[m], own[0], r[m])));
+                    r[m] = JSEL ? (jterm ? keep(t) : r[m]) : t;
+                }
+            };
+            auto kpart = [&]() {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const T t = __builtin_fmaf(-c.kn[m + 6], nb[2],
+                                __builtin_fmaf(-c.kn[m + 3], nb[1],
+                                __builtin_fmaf(-c.kn[m], nb[0], r[m])));
+                    r[m] = KSEL ? (kterm ? keep(t) : r[m]) : t;
+                }
+            };
+            if constexpr (FWD) {
+                jpart();
+                kpart();
+            } else {
+                kpart();
+                jpart();
+            }
+            lu_solve3_fma(c.f, r[0], r[1], r[2], o);
+        } else if constexpr (NEQ == 3) {
             double r[3] = {c.r[0], c.r[1], c.r[2]};
             auto jpart = [&]() {
 #pragma unroll
@@ -493,7 +592,7 @@ struct Band {
         }
         st_vec<NEQ>(o, rO, vo, pstride);   // inactive positions are padding
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) edge_st(o[m], rEout, vout + m * 8, eout);
+        for (int m = 0; m < NEQ; ++m) edge_st((double)o[m], rEout, vout + m * 8, eout);
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) own[m] = o[m];
         load(c, vp, ein_pf);
@@ -515,11 +614,11 @@ struct Band {
     }
 };
 
-template <int NEQ, int PF, bool FWD>
+template <int NEQ, int PF, bool FWD, typename T>
 __global__ void __launch_bounds__(BAND_LANES)
 k_band(Dims d, LusgsWs w)
 {
-    constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ>();
+    constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ, T>();
     const int lane = threadIdx.x, nband = gridDim.x;
     // backward bands are numbered top down so that workgroups start in dependency order
     const int b = FWD ? (int)blockIdx.x : nband - 1 - (int)blockIdx.x;
@@ -531,7 +630,7 @@ k_band(Dims d, LusgsWs w)
     const int dir = FWD ? 1 : -1;
     const int h0 = FWD ? hbeg : hend;
 
-    Band<NEQ, PF, FWD> B;
+    Band<NEQ, PF, FWD, T> B;
     B.d = d;
     B.k = real ? k0 + lane : w.ks - 1;          // idle lanes run on a padding row
     // hand-over rows: the consumer row takes its k-neighbour from the next band over
@@ -555,9 +654,9 @@ k_band(Dims d, LusgsWs w)
     // the k-neighbour term is not formed on the first (FWD) / last grid row
     B.kterm = FWD ? B.k > 0 : B.k < d.km - 1;
 
-    B.pstride = (unsigned)(w.ns * PW * 8);                                // bytes per plane pair
-    const unsigned gbytes = (unsigned)pairs_of<NEQ>(NB) * B.pstride;     // bytes per block group
-    const unsigned vbytes = (unsigned)pairs_of<NEQ>(NEQ) * B.pstride;
+    B.pstride = (unsigned)(w.ns * PW * sizeof(T));                      // bytes per plane group
+    const unsigned gbytes = (unsigned)pairs_of<NEQ, T>(NB) * B.pstride;  // bytes per block group
+    const unsigned vbytes = (unsigned)pairs_of<NEQ, T>(NEQ) * B.pstride;
     const char *lhs = (const char *)w.lhs;
     // The last hyperplane of every plane is never touched (sweeps reach HP0 - 1 past
     // their ends), so each range may stop a few bytes short of its group; distinct ranges
@@ -567,15 +666,15 @@ k_band(Dims d, LusgsWs w)
     B.rK = mk_rsrc(lhs + (FWD ? 2 : 4) * (size_t)gbytes, gbytes - 48);
     B.rR = mk_rsrc(w.r, vbytes - 64);
     B.rO = FWD ? B.rR : mk_rsrc(w.dq, vbytes - 80);
-    B.dstep = dir * w.ks * PW * 8;
-    B.vo = (unsigned)(((long)(h0 + HP0) * w.ks + B.k) * PW * 8);   // current step
+    B.dstep = dir * w.ks * PW * (int)sizeof(T);
+    B.vo = (unsigned)(((long)(h0 + HP0) * w.ks + B.k) * PW * sizeof(T));   // current step
     B.vp = B.vo + PF * B.dstep;                                     // prefetch step
     B.h = h0;
     B.eout = (unsigned)(h0 - B.kout + EPAD) * 32u;
     const unsigned ein0 = (unsigned)(h0 - B.kin + EPAD) * 32u;
     B.ein_pf = ein0 + PF * dir * 32;
 #pragma unroll
-    for (int m = 0; m < NEQ; ++m) B.own[m] = 0.0;
+    for (int m = 0; m < NEQ; ++m) B.own[m] = (T)0;
 #pragma unroll
     for (int u = 0; u < PF; ++u) B.load(B.s[u], B.vo + u * B.dstep, ein0 + u * dir * 32);
 
@@ -611,20 +710,21 @@ void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &w)
     w.nedge = (2L * w.nband + 1) * w.epitch;
 }
 
+// sizes for the FP64 system (the FP32-LHS one needs less and fits the same allocation)
 long lusgs_ws_lhs_bytes(const LusgsWs &w, int neqs)
 {
-    return neqs == 3 ? 5L * pairs_of<3>(9) * w.ns * 2 * 8 : 5L * w.ns * 8;
+    return neqs == 3 ? 5L * pairs_of<3, double>(9) * w.ns * 16 : 5L * w.ns * 8;
 }
 
 long lusgs_ws_vec_bytes(const LusgsWs &w, int neqs)
 {
-    return neqs == 3 ? (long)pairs_of<3>(3) * w.ns * 2 * 8 : w.ns * 8;
+    return neqs == 3 ? (long)pairs_of<3, double>(3) * w.ns * 16 : w.ns * 8;
 }
 
 bool lusgs_ws_fits(const LusgsWs &w)
 {
     // buffer offsets are 32-bit: one block group (5 plane pairs) must stay below 2 GiB
-    return (long)pairs_of<3>(9) * w.ns * 2 * 8 < (1L << 31);
+    return (long)pairs_of<3, double>(9) * w.ns * 16 < (1L << 31);
 }
 
 template <typename LT, int NEQ>
@@ -637,7 +737,7 @@ static void launch_lusgs(hipStream_t sm, const Dims &d, const LusgsWs &w, const 
                        bkm, bjp, bkp, fac);
     const int pf = NEQ == 3 ? w.pf3 : w.pf1;
 #define BAND_LAUNCH(PFV, FWDV)                                                                 \
-    hipLaunchKernelGGL((k_band<NEQ, PFV, FWDV>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
+    hipLaunchKernelGGL((k_band<NEQ, PFV, FWDV, LT>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
 #define BAND_SWITCH(FWDV)                                                                      \
     do {                                                                                       \
         switch (pf) {                                                                          \
@@ -656,9 +756,11 @@ static void launch_lusgs(hipStream_t sm, const Dims &d, const LusgsWs &w, const 
     BAND_SWITCH(false);
 #undef BAND_SWITCH
 #undef BAND_LAUNCH
-    hipLaunchKernelGGL((k_unshear<NEQ>), tg, dim3(TILE_THREADS), 0, sm, d, w, w.dq, dq);
+    hipLaunchKernelGGL((k_unshear<NEQ, LT>), tg, dim3(TILE_THREADS), 0, sm, d, w,
+                       (const LT *)w.dq, dq);
     if (rhs_out)
-        hipLaunchKernelGGL((k_unshear<NEQ>), tg, dim3(TILE_THREADS), 0, sm, d, w, w.r, rhs);
+        hipLaunchKernelGGL((k_unshear<NEQ, LT>), tg, dim3(TILE_THREADS), 0, sm, d, w,
+                           (const LT *)w.r, rhs);
     launch_wake(sm, d, NEQ, wk, q, dq, rhs);
 }
 
@@ -682,5 +784,9 @@ template void launch_lusgs3<double>(hipStream_t, const Dims &, const LusgsWs &, 
                                     const double *, double *, double *, const double *,
                                     const double *, const double *, const double *,
                                     const double *, double *, bool);
+template void launch_lusgs3<float>(hipStream_t, const Dims &, const LusgsWs &, const WakePairs &,
+                                   const double *, double *, double *, const float *,
+                                   const float *, const float *, const float *, const float *,
+                                   float *, bool);
 
 }  // namespace smac

@@ -66,8 +66,8 @@ struct LusgsWs {
     int epitch;                  // slots per band region: (jm + 2*128 padding)*4
     long nedge;                  // (2*nband + 1)*epitch slots
     void *lhs;                   // 5 block groups (factors, bjm, bkm, bjp, bkp), LT
-    double *r;                   // neqs planes: rhs, forward result, middle result
-    double *dq;                  // neqs planes
+    void *r;                     // neqs planes: rhs, forward result, middle result
+    void *dq;                    // neqs planes (the solve's type: FP64, or FP32 for FP32-LHS)
     unsigned long long *edge;    // band hand-over slots (forward half, backward half)
     int *stall;                  // raised when a hand-over wait times out
     int pf3, pf1;                // prefetch distance (steps) of the 3x3 and scalar sweeps
