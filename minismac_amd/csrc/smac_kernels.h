@@ -71,6 +71,7 @@ struct LusgsWs {
     unsigned long long *edge;    // band hand-over slots (forward half, backward half)
     int *stall;                  // raised when a hand-over wait times out
     int pf3, pf1;                // prefetch distance (steps) of the 3x3 and scalar sweeps
+    hipEvent_t ev_fwd[2];        // when set: recorded around the forward band sweep
 };
 void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &);
 long lusgs_ws_lhs_bytes(const LusgsWs &, int neqs);   // the five block groups

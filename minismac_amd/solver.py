@@ -128,7 +128,7 @@ class Solver:
     def timings(self) -> dict:
         out = (C.c_double * 7)()
         _check(lib().smac_timings(self.h, out), "smac_timings")
-        keys = ["sa", "assemble", "resid", "solve", "update_force", "unused", "pass"]
+        keys = ["sa", "assemble", "resid", "solve", "update_force", "band_fwd", "pass"]
         return dict(zip(keys, list(out)))
 
     def array(self, name: str) -> np.ndarray:
