@@ -1,18 +1,13 @@
 """Python access to the oracle -- TEST INFRASTRUCTURE ONLY.
 
-Two checkers live here:
-  * ``Oracle``: the C restatement (oracle/_ref/libsmac_oracle.so) driven through ctypes;
-  * ``run_reference``: the compiled reference (oracle/_ref/smac2d_ref), run on a case's
-    input files with the instrumented seams, returning its per-iteration trace.
-Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this module.
+``Oracle`` drives the C restatement (oracle/_ref/libsmac_oracle.so) through ctypes;
+``lusgs`` / ``lrelax`` run its stand-alone solvers on caller arrays.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg use this module.
 """
 from __future__ import annotations
 
 import ctypes as C
 import os
-import resource
-import subprocess
-import tempfile
 
 import numpy as np
 
@@ -22,8 +17,6 @@ from minismac_amd.case import Case, dptr, fortran
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 LIB_PATH = os.path.join(REF_DIR, "libsmac_oracle.so")
-REF_EXE = os.path.join(REF_DIR, "smac2d_ref")
-REF_EXE_FAST = os.path.join(REF_DIR, "smac2d_ref_fast")
 
 _lib = None
 
@@ -160,87 +153,38 @@ def metric(x: np.ndarray, y: np.ndarray):
 # the compiled reference
 
 
-def _unlimited_stack():
-    # flang places the reference's automatic arrays on the stack
-    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+def _pairs(wake_pairs):
+    if wake_pairs is None or len(wake_pairs) == 0:
+        return 0, None
+    w = np.ascontiguousarray(np.asarray(wake_pairs, dtype=np.int32).reshape(-1))
+    return len(w) // 2, w
 
 
-def parse_trace(path: str) -> dict:
-    """STEP/NORM/FORCE records of oracle/ref_trace.c -> dict of numpy arrays by nt."""
-    steps, norms, forces = {}, {}, {}
-    for line in open(path):
-        t = line.split()
-        if not t:
-            continue
-        if t[0] == "STEP":
-            nt = int(t[1])
-            steps[nt] = dict(resmax0=float(t[2]), resmax=float(t[3]), jres=int(t[4]),
-                             kres=int(t[5]), divmax=float(t[6]), turres=float(t[7]),
-                             clift_in=float(t[8]))
-        elif t[0] == "NORM":
-            norms[int(t[1])] = float(t[2])
-        elif t[0] == "FORCE":
-            forces[int(t[1])] = tuple(float(v) for v in t[2:8])
-    nts = sorted(steps)
-    out = {"nt": np.array(nts)}
-    for key in ("resmax0", "resmax", "jres", "kres", "divmax", "turres", "clift_in"):
-        out[key] = np.array([steps[n][key] for n in nts])
-    out["rhsnorm"] = np.array([norms.get(n, np.nan) for n in nts])
-    f = np.array([forces[n] for n in nts]) if forces else np.zeros((0, 6))
-    if len(f):
-        alpha, fxp, fyp, fxs, fys, fmom = f.T
-        pi = 4.0 * np.arctan(1.0)
-        ca, sa = np.cos(alpha * pi / 180.0), np.sin(alpha * pi / 180.0)
-        # coefficient formulas of ref/smac2d.f:1737-1741 (chord = 1)
-        out["cd"] = (2.0 * (fxp + fxs) * ca + 2.0 * (fyp + fys) * sa) / 1.0
-        out["cl"] = (2.0 * (fyp + fys) * ca - 2.0 * (fxp + fxs) * sa) / 1.0
-        out["cm"] = -(2.0 * fmom / 1.0)
-        out["fxp"], out["fyp"], out["fxs"], out["fys"], out["fmom"] = fxp, fyp, fxs, fys, fmom
-    return out
+def lusgs(params: SmacParams, neqs: int, q, dq, rhs, btc, bjm, bjp, bkm, bkp, scr1,
+          wake_pairs=None) -> None:
+    """orc_lusgs on Fortran-ordered float64 arrays (modified in place); wake_pairs are the
+    1-based (jt, jb) pairs."""
+    jmax, kmax = q.shape[0], q.shape[1]
+    n, w = _pairs(wake_pairs)
+    arrs = [q, dq, rhs, btc, bjm, bjp, bkm, bkp, scr1]
+    for a in arrs:
+        assert a.flags.f_contiguous and a.dtype == np.float64
+    rc = lib().orc_lusgs(C.byref(params), jmax, kmax, neqs, n,
+                         w.ctypes.data_as(C.POINTER(C.c_int)) if w is not None else None,
+                         *[dptr(a) for a in arrs])
+    if rc:
+        raise RuntimeError(f"orc_lusgs returned {rc}")
 
 
-def run_reference(case: Case, ntmax: int, workdir: str | None = None, dump_at=(),
-                  fast: bool = False, timeout: float = 3600.0) -> dict:
-    """Run the compiled reference on `case` for ntmax iterations; returns the parsed trace
-    (plus 'stdout', 'dir', and 'solver_seconds' from its TIMER line)."""
-    exe = REF_EXE_FAST if fast else REF_EXE
-    if not os.path.exists(exe):
-        raise FileNotFoundError(f"{exe} missing: run `make -C oracle ref`")
-    d = workdir or tempfile.mkdtemp(prefix="smac_ref_")
-    case.write_reference_inputs(d, ntmax)
-    env = dict(os.environ)
-    env["SMAC_TRACE"] = os.path.join(d, "trace.txt")
-    if dump_at:
-        env["SMAC_DUMP"] = ",".join(str(n) for n in dump_at)
-        env["SMAC_DUMP_DIR"] = d
-    r = subprocess.run([exe], cwd=d, env=env, preexec_fn=_unlimited_stack, capture_output=True,
-                       text=True, timeout=timeout, errors="replace")
-    if r.returncode != 0:
-        raise RuntimeError(f"reference run failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
-    out = parse_trace(env["SMAC_TRACE"]) if not fast else {}
-    out["stdout"] = r.stdout
-    out["dir"] = d
-    for line in r.stdout.splitlines():
-        if "TIMER: final solver time (sec)" in line:
-            out["solver_seconds"] = float(line.split("=")[1])
-    return out
-
-
-def load_dump(directory: str, what: str, nt: int, jmax: int, kmax: int) -> dict:
-    a = np.fromfile(os.path.join(directory, f"{what}_{nt}.bin"), dtype="<f8")
-    n = jmax * kmax
-    layout = {"qin": [("q", 3), ("anut", 1)],
-              "qout": [("q", 3), ("anut", 1), ("vnut", 1), ("dq", 3)],
-              "lhs": [("q", 3), ("s", 3), ("btc", 9), ("bjm", 9), ("bjp", 9), ("bkm", 9), ("bkp", 9)],
-              "dq": [("dq", 3), ("s", 3)],
-              "sa": [("anut", 1), ("sf", 1), ("sjm", 1), ("sjp", 1), ("skm", 1), ("skp", 1),
-                     ("stc", 1)],
-              "sads": [("ds", 1)],
-              "sascr": [("vort", 1), ("diff", 1), ("conv", 1), ("prod", 1), ("dest", 1),
-                        ("fr1", 1), ("tran", 1)]}[what]
-    out, off = {}, 0
-    for name, planes in layout:
-        shape = (jmax, kmax) + ((planes,) if planes > 1 else ())
-        out[name] = a[off:off + n * planes].reshape(shape, order="F")
-        off += n * planes
-    return out
+def lrelax(params: SmacParams, q, dq, s, btc, bjm, bjp, bkm, bkp, wake_pairs=None) -> None:
+    """orc_lrelax on Fortran-ordered float64 arrays (modified in place)."""
+    jmax, kmax = q.shape[0], q.shape[1]
+    n, w = _pairs(wake_pairs)
+    arrs = [q, dq, s, btc, bjm, bjp, bkm, bkp]
+    for a in arrs:
+        assert a.flags.f_contiguous and a.dtype == np.float64
+    rc = lib().orc_lrelax(C.byref(params), jmax, kmax, n,
+                          w.ctypes.data_as(C.POINTER(C.c_int)) if w is not None else None,
+                          *[dptr(a) for a in arrs])
+    if rc:
+        raise RuntimeError(f"orc_lrelax returned {rc}")

@@ -4,9 +4,14 @@
  * The oracle is the checker for the gfx950 path: tests, __graft_entry__.smoke() and
  * bench.py's CPU-baseline leg may load it; the product never does.  It restates the
  * reference routines one to one (file:line citations in smac_oracle.c) in plain C99
- * built with -O2 -ffp-contract=off, so its arithmetic is the reference's operation for
- * operation.  It is pinned against the compiled reference (oracle/_ref, per-iteration
- * trace at full precision) by tests/test_oracle_vs_ref.py.
+ * built with -O2 -ffp-contract=off, keeping the reference's operation order.  The
+ * reference itself cannot be built in this image (its sources include mpif.h; there is no
+ * MPI), so the restatement is pinned by tests/test_oracle_pinning.py against what the
+ * mathematics fixes (dense solves of the factorised systems, factor identities, the
+ * wake-cut mean, freestream preservation and the block-sum identity, brute-force wall
+ * distance, first-index argmax) and by tests/test_grid_metric.py (exact metrics of affine
+ * grids).  Parity unpinned: the SA source assembly (tsafill), the far-field point-vortex
+ * rows and the wall normals of the pressure force.
  *
  * Layout: the reference's Fortran arrays (j fastest; see include/smac_amd.h).
  */

@@ -1,0 +1,377 @@
+"""Pins the oracle (the CPU restatement) against what the mathematics fixes.
+
+The reference itself cannot be built here: every file of its implicit iteration includes
+mpif.h and the image has no MPI.  So each part of the restatement is checked against an
+independent statement of what it must compute -- dense solves, factorisation identities,
+conservation and invariance properties -- chosen so that a dropped term, a wrong sign or
+index, or a transposed block fails:
+
+  lusini3 / lusini1    the factors multiply back to the diagonal block (L*U = btc)
+  ilusol3 / ilusol1    dq solves the approximate factorisation (D+L) D^-1 (D+U) dq = rhs,
+                       built densely
+  btlu3 + line sweeps  uncoupled lines: each line's block-tridiagonal system solved densely;
+                       coupled: block Gauss-Seidel over lines with dense line solves,
+                       alternating directions and under-relaxation
+  bcimpds              after the solve the wake-cut point carries the mean of its two
+                       neighbours across the cut: (q+dq)(jt,1) = ((q+dq)(jt,2)+(q+dq)(jb,2))/2
+  conup+flux+visc+diag uniform flow on a uniform grid: zero residual, and the five blocks
+                       summing to the pseudo-time diagonal I/(J dtau) (the flux-split parts
+                       telescope, the viscous parts cancel)
+  dfunsa               wall distance = brute-force minimum over the wall nodes
+  force/fmk            no motion: zero shear force; uniform pressure: force and moment
+                       exactly linear in p (the discrete normals themselves are not pinned)
+  isamax               ties resolve to the first index in Fortran order
+Not pinned here (parity unpinned): the SA source assembly (tsafill), the far-field
+point-vortex rows (bc 25) and the wall-normal discretisation of the pressure force have no
+closed form that the checks above reach.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from minismac_amd._abi import make_params
+from minismac_amd.case import Case, config_case
+from minismac_amd import grid
+
+RNG = np.random.default_rng(20261202)
+
+
+def _blocks(jm, km, diag=4.0, off=0.3):
+    """Random diagonally dominant 3x3 block fields in the reference layout (m + 3n)."""
+    def field(scale):
+        return np.asfortranarray(RNG.uniform(-scale, scale, (jm, km, 9)))
+    btc = field(off)
+    for m in range(3):
+        btc[:, :, m + 3 * m] += diag
+    return btc, field(off), field(off), field(off), field(off)
+
+
+def _blk(a, j, k):
+    return a[j, k, :].reshape(3, 3, order="F")
+
+
+def _dense(jm, km, btc, bjm, bjp, bkm, bkp, parts=("d", "l", "u")):
+    n = 3 * jm * km
+    A = np.zeros((n, n))
+    idx = lambda j, k: 3 * (j + jm * k)
+    for k in range(km):
+        for j in range(jm):
+            r = idx(j, k)
+            if "d" in parts:
+                A[r:r + 3, r:r + 3] += _blk(btc, j, k)
+            if "l" in parts:
+                if j > 0:
+                    A[r:r + 3, idx(j - 1, k):idx(j - 1, k) + 3] += _blk(bjm, j, k)
+                if k > 0:
+                    A[r:r + 3, idx(j, k - 1):idx(j, k - 1) + 3] += _blk(bkm, j, k)
+            if "u" in parts:
+                if j < jm - 1:
+                    A[r:r + 3, idx(j + 1, k):idx(j + 1, k) + 3] += _blk(bjp, j, k)
+                if k < km - 1:
+                    A[r:r + 3, idx(j, k + 1):idx(j, k + 1) + 3] += _blk(bkp, j, k)
+    return A
+
+
+def _vec(a):
+    """(jm,km,3) Fortran array -> point-major vector (3 per point, j fastest)."""
+    return np.transpose(a, (2, 0, 1)).reshape(-1, order="F")
+
+
+# ------------------------------------------------------------------------------------------
+# LU-SGS
+
+def test_lusini3_factors_multiply_back():
+    jm, km = 6, 5
+    btc, bjm, bjp, bkm, bkp = _blocks(jm, km)
+    bjm[0] = 0.0
+    bjp[-1] = 0.0
+    q = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    dq = np.zeros_like(q)
+    rhs = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    scr1 = np.zeros_like(btc)
+    pyoracle.lusgs(make_params(), 3, q, dq, rhs, btc.copy(order="F"), bjm, bjp, bkm, bkp, scr1)
+    for k in range(km):
+        for j in range(jm):
+            f = scr1[j, k]
+            L = np.array([[1 / f[0], 0, 0], [f[1], 1 / f[4], 0], [f[2], f[5], 1 / f[8]]])
+            U = np.array([[1, f[3], f[6]], [0, 1, f[7]], [0, 0, 1]])
+            np.testing.assert_allclose(L @ U, _blk(btc, j, k), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("jm,km", [(5, 4), (7, 6), (9, 3)])
+def test_lusgs3_solves_approximate_factorisation(jm, km):
+    btc, bjm, bjp, bkm, bkp = _blocks(jm, km)
+    bjm[0] = 0.0       # the C-grid rows the hyperplane order relies on
+    bjp[-1] = 0.0
+    q = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    rhs0 = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    dq = np.zeros_like(q)
+    rhs = rhs0.copy(order="F")
+    scr1 = np.zeros_like(btc)
+    pyoracle.lusgs(make_params(), 3, q, dq, rhs, btc.copy(order="F"), bjm, bjp, bkm, bkp, scr1)
+    D = _dense(jm, km, btc, bjm, bjp, bkm, bkp, parts=("d",))
+    DL = _dense(jm, km, btc, bjm, bjp, bkm, bkp, parts=("d", "l"))
+    DU = _dense(jm, km, btc, bjm, bjp, bkm, bkp, parts=("d", "u"))
+    M = DL @ np.linalg.solve(D, DU)
+    r = M @ _vec(dq) - _vec(rhs0)
+    assert np.abs(r).max() <= 1e-12 * np.abs(_vec(rhs0)).max()
+    # a plausible slip -- the full operator instead of the factorisation -- fails
+    A = _dense(jm, km, btc, bjm, bjp, bkm, bkp)
+    assert np.abs(A @ _vec(dq) - _vec(rhs0)).max() > 1e-6
+
+
+def test_lusgs1_scalar_solves_approximate_factorisation():
+    jm, km = 7, 5
+    b = lambda s: np.asfortranarray(RNG.uniform(-s, s, (jm, km)))
+    btc = b(0.3) + 3.0
+    bjm, bjp, bkm, bkp = b(0.4), b(0.4), b(0.4), b(0.4)
+    bjm[0] = 0.0
+    bjp[-1] = 0.0
+    q = np.asfortranarray(RNG.normal(size=(jm, km)))
+    rhs0 = np.asfortranarray(RNG.normal(size=(jm, km)))
+    dq = np.zeros_like(q)
+    rhs = rhs0.copy(order="F")
+    scr1 = np.zeros_like(btc)
+    pyoracle.lusgs(make_params(), 1, q, dq, rhs, btc.copy(order="F"), bjm, bjp, bkm, bkp, scr1)
+    np.testing.assert_allclose(scr1, 1.0 / btc, rtol=1e-15)
+    n = jm * km
+    D = np.diag(btc.ravel(order="F"))
+    L = np.zeros((n, n))
+    U = np.zeros((n, n))
+    for k in range(km):
+        for j in range(jm):
+            p = j + jm * k
+            if j > 0:
+                L[p, p - 1] = bjm[j, k]
+            if k > 0:
+                L[p, p - jm] = bkm[j, k]
+            if j < jm - 1:
+                U[p, p + 1] = bjp[j, k]
+            if k < km - 1:
+                U[p, p + jm] = bkp[j, k]
+    M = (D + L) @ np.linalg.solve(D, D + U)
+    r = M @ dq.ravel(order="F") - rhs0.ravel(order="F")
+    assert np.abs(r).max() <= 1e-12 * np.abs(rhs0).max()
+
+
+def test_lusgs_wake_cut_takes_mean_across_the_cut():
+    case = config_case(0, impsch=2)
+    o = pyoracle.Oracle(case)
+    for _ in range(3):
+        o.step()
+    o.assemble()
+    q = o.array("q")
+    o.solve()
+    qn = q + o.array("dq")
+    pairs = o.wake_pairs()
+    assert len(pairs) > 0
+    for jt, jb in pairs:
+        want = 0.5 * (qn[jt - 1, 1] + qn[jb - 1, 1])
+        np.testing.assert_allclose(qn[jt - 1, 0], want, rtol=0, atol=1e-13)
+
+
+# ------------------------------------------------------------------------------------------
+# line relaxation
+
+def _params(**kw):
+    return make_params(**kw)
+
+
+def test_lrelax_uncoupled_lines_are_solved_exactly():
+    jm, km = 8, 5
+    btc, bjm, bjp, bkm, bkp = _blocks(jm, km)
+    bkm[:] = 0.0
+    bkp[:] = 0.0
+    q = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    s = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    dq = np.zeros_like(q)
+    p = _params(njsp=1, nksp=0, underr=1.0)
+    pyoracle.lrelax(p, q, dq, s.copy(order="F"), btc.copy(order="F"), bjm, bjp.copy(order="F"),
+                    bkm, bkp.copy(order="F"))
+    for k in range(km):
+        T = np.zeros((3 * jm, 3 * jm))
+        for j in range(jm):
+            T[3 * j:3 * j + 3, 3 * j:3 * j + 3] = _blk(btc, j, k)
+            if j > 0:
+                T[3 * j:3 * j + 3, 3 * j - 3:3 * j] = _blk(bjm, j, k)
+            if j < jm - 1:
+                T[3 * j:3 * j + 3, 3 * j + 3:3 * j + 6] = _blk(bjp, j, k)
+        x = np.linalg.solve(T, s[:, k, :].reshape(-1))
+        np.testing.assert_allclose(dq[:, k, :].reshape(-1), x, rtol=1e-12, atol=1e-13)
+
+
+def _dense_line_gs(p, s, btc, bjm, bjp, bkm, bkp):
+    """Block line Gauss-Seidel with dense line solves: njsp xi sweeps (odd ascending in k,
+    even descending), then nksp eta sweeps (likewise in j); each line update relaxed by
+    underr; the neighbour-line terms use the latest dq."""
+    jm, km = s.shape[0], s.shape[1]
+    dq = np.zeros_like(s)
+
+    def line(blocks_diag, blocks_lo, blocks_hi, rhs):
+        n = len(rhs)
+        T = np.zeros((3 * n, 3 * n))
+        for i in range(n):
+            T[3 * i:3 * i + 3, 3 * i:3 * i + 3] = blocks_diag[i]
+            if i > 0:
+                T[3 * i:3 * i + 3, 3 * i - 3:3 * i] = blocks_lo[i]
+            if i < n - 1:
+                T[3 * i:3 * i + 3, 3 * i + 3:3 * i + 6] = blocks_hi[i]
+        return np.linalg.solve(T, np.concatenate(rhs)).reshape(n, 3)
+
+    for sweep in range(1, p.njsp + 1):
+        ks = range(km) if sweep % 2 else range(km - 1, -1, -1)
+        for k in ks:
+            rhs = []
+            for j in range(jm):
+                r = s[j, k].copy()
+                if k > 0:
+                    r -= _blk(bkm, j, k) @ dq[j, k - 1]
+                if k < km - 1:
+                    r -= _blk(bkp, j, k) @ dq[j, k + 1]
+                rhs.append(r)
+            x = line([_blk(btc, j, k) for j in range(jm)], [_blk(bjm, j, k) for j in range(jm)],
+                     [_blk(bjp, j, k) for j in range(jm)], rhs)
+            dq[:, k] += p.underr * (x - dq[:, k])
+    for sweep in range(1, p.nksp + 1):
+        js = range(jm) if sweep % 2 else range(jm - 1, -1, -1)
+        for j in js:
+            rhs = []
+            for k in range(km):
+                r = s[j, k].copy()
+                if j > 0:
+                    r -= _blk(bjm, j, k) @ dq[j - 1, k]
+                if j < jm - 1:
+                    r -= _blk(bjp, j, k) @ dq[j + 1, k]
+                rhs.append(r)
+            x = line([_blk(btc, j, k) for k in range(km)], [_blk(bkm, j, k) for k in range(km)],
+                     [_blk(bkp, j, k) for k in range(km)], rhs)
+            dq[j, :] += p.underr * (x - dq[j, :])
+    return dq
+
+
+@pytest.mark.parametrize("njsp,nksp,underr", [(2, 2, 0.7), (1, 1, 1.0), (3, 0, 0.4)])
+def test_lrelax_is_block_line_gauss_seidel(njsp, nksp, underr):
+    jm, km = 7, 6
+    btc, bjm, bjp, bkm, bkp = _blocks(jm, km)
+    q = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    s = np.asfortranarray(RNG.normal(size=(jm, km, 3)))
+    dq = np.zeros_like(q)
+    p = _params(njsp=njsp, nksp=nksp, underr=underr)
+    pyoracle.lrelax(p, q, dq, s.copy(order="F"), btc.copy(order="F"), bjm, bjp.copy(order="F"),
+                    bkm, bkp.copy(order="F"))
+    want = _dense_line_gs(p, s, btc, bjm, bjp, bkm, bkp)
+    np.testing.assert_allclose(dq, want, rtol=1e-11, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------
+# assembly
+
+def _box_case(jm=24, km=14, **kw):
+    """Uniform Cartesian grid with the C-grid BC row types on its edges."""
+    x, y = np.meshgrid(np.linspace(0.0, 2.3, jm), np.linspace(0.0, 1.1, km), indexing="ij")
+    regs = [(31, 1, 1, 1, 1, km), (31, 1, jm, jm, 1, km), (0, 1, 1, jm, 1, 1), (25, 1, 1, jm, km, km)]
+    p = make_params(**kw)
+    return Case(x=np.asfortranarray(x), y=np.asfortranarray(y), regions=regs, params=p,
+                name=f"box {jm}x{km}")
+
+
+@pytest.mark.parametrize("ivis", [0, 1])
+def test_uniform_flow_has_zero_residual_and_blocks_sum_to_the_diagonal(ivis):
+    case = _box_case(ivis=ivis, iturb=0, dtau=0.37, beta=7.0)
+    o = pyoracle.Oracle(case)
+    jm, km = case.jmax, case.kmax
+    q = np.zeros((jm, km, 3), order="F")
+    q[:, :, 0] = 1.3
+    q[:, :, 1] = 0.8
+    q[:, :, 2] = -0.25
+    o.set_array("q", q)
+    o.assemble()
+    s = o.array("s")
+    inner = (slice(1, jm - 1), slice(1, km - 1))
+    assert np.abs(s[inner]).max() == 0.0
+    dj = o.array("dj")
+    blocks = [o.array("lhs_" + n) for n in ("btc", "bjm", "bjp", "bkm", "bkp")]
+    tot = sum(blocks)
+    scale = max(np.abs(b[inner]).max() for b in blocks)
+    want = np.zeros_like(tot)
+    for m in range(3):
+        want[:, :, m + 3 * m] = 1.0 / (dj * case.params.dtau)
+    assert np.abs(tot[inner] - want[inner]).max() <= 1e-13 * scale
+    # and the single blocks are not trivially zero
+    assert np.abs(blocks[1][inner]).max() > 1e-3 * scale
+
+
+# ------------------------------------------------------------------------------------------
+# SA wall distance, forces, argmax
+
+def test_wall_distance_is_the_brute_force_minimum():
+    case = config_case(0, impsch=1)
+    o = pyoracle.Oracle(case)
+    o.sa_assemble()
+    smin = o.array("smin")
+    x, y = case.x, case.y
+    walls = [r for r in case.resolved_regions() if r[0] == 0]
+    wx = np.concatenate([x[r[1] - 1:r[2], r[3] - 1] for r in walls])
+    wy = np.concatenate([y[r[1] - 1:r[2], r[3] - 1] for r in walls])
+    d = np.sqrt((x[:, :, None] - wx) ** 2 + (y[:, :, None] - wy) ** 2).min(axis=2)
+    np.testing.assert_allclose(smin, d, rtol=1e-14, atol=1e-15)
+
+
+def test_wall_force_at_rest_and_its_linearity_in_pressure():
+    """No motion: the shear parts vanish exactly.  Uniform pressure: the pressure force and
+    moment are p times those of unit pressure, exactly (a power-of-two scaling).  (Zero net
+    pressure force is not a property of this discretisation: the face-averaged node normals
+    leave a trailing-edge term of the open wall chain.)"""
+    case = config_case(0, impsch=1)
+    jm, km = case.jmax, case.kmax
+    out = []
+    for p in (1.0, 2.0):
+        o = pyoracle.Oracle(case)
+        q = np.zeros((jm, km, 3), order="F")
+        q[:, :, 0] = p
+        o.set_array("q", q)
+        o.set_array("dq", np.zeros_like(q))
+        out.append(o.finish())
+    for st in out:
+        assert st["fxs"] == 0.0 and st["fys"] == 0.0
+    for k in ("fxp", "fyp", "fmom"):
+        assert out[1][k] == 2.0 * out[0][k]
+
+
+def test_residual_argmax_takes_first_index_and_matches_brute_force():
+    # ties: free stream at the outflow pressure on a box with outflow, wake-cut and far-field
+    # rows gives s = 0 everywhere -> the first point (1,1)
+    jm, km, h = 24, 14, 12
+    x, y = np.meshgrid(np.linspace(0.0, 2.3, jm), np.linspace(0.0, 1.1, km), indexing="ij")
+    regs = [(31, 1, 1, 1, 1, km), (31, 1, jm, jm, 1, km), (60, 1, 1, h, 1, 1),
+            (60, 1, h + 1, jm, 1, 1), (25, 1, 1, jm, km, km)]
+    case = Case(x=np.asfortranarray(x), y=np.asfortranarray(y), regions=regs,
+                params=make_params(alpha=0.0, iturb=0, pout=1.0), name="box")
+    o = pyoracle.Oracle(case)
+    q = np.zeros((jm, km, 3), order="F")
+    q[:, :, 0] = 1.0
+    q[:, :, 1] = 1.0
+    o.set_array("q", q)
+    st = o.assemble()
+    assert np.abs(o.array("s")).max() == 0.0
+    assert st["resmax"] == 0.0 and (st["jres"], st["kres"]) == (1, 1)
+    # a real state: the location of max |s*dj| over (j, k, equation), first occurrence
+    case = config_case(0, impsch=1)
+    o = pyoracle.Oracle(case)
+    for _ in range(4):
+        o.step()
+    st = o.assemble()
+    r = np.abs(o.array("s") * o.array("dj")[:, :, None])
+    i = int(np.argmax(r.ravel(order="F")))
+    j, k, _ = np.unravel_index(i, r.shape, order="F")
+    assert (st["jres"], st["kres"]) == (j + 1, k + 1)
+    assert st["resmax"] == r.ravel(order="F")[i]
+
+
+def test_wake_pairs_mirror_across_the_cut():
+    case = config_case(0)
+    o = pyoracle.Oracle(case)
+    pairs = o.wake_pairs()
+    jm = case.jmax
+    assert all(jb == jm + 1 - jt for jt, jb in pairs)
+    assert sorted(set(pairs[:, 0])) == sorted(set(pairs[:, 1]))
