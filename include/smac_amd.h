@@ -127,7 +127,7 @@ smac_status smac_timings(smac_handle *h, double out[7]);
 /* Options: "keep_lhs" (1: keep a copy of the five assembled blocks of each pass for
  * smac_get_array), "timing" (1: time kernel families with HIP events on the handle's
  * stream), "lusgs_rows" (1..64, default 64: grid rows per wavefront band of the LU-SGS
- * sweeps), "lusgs_prefetch" / "lusgs_prefetch_sa" (1, 2, 3, 4, 6 or 8; defaults 3 and 8:
+ * sweeps), "lusgs_prefetch" / "lusgs_prefetch_sa" (1, 2, 3, 4, 6 or 8; defaults 4 and 8:
  * steps of operand prefetch in the 3x3 / scalar LU-SGS sweeps).  The last three are tuning
  * knobs; results do not depend on them.  No counterpart in the reference. */
 smac_status smac_set_option(smac_handle *h, const char *name, int value);

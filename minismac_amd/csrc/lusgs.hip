@@ -524,7 +524,12 @@ struct Band {
                     const T t = __builtin_fmaf(-c.jn[m + 6], own[2],
                                 __builtin_fmaf(-c.jn[m + 3], own[1],
                                 __builtin_fmaf(-c.jn[m], own[0], r[m])));
-                    r[m] = JSEL ? (jterm ? keep(t) : r[m]) : t;
+                    if constexpr (JSEL) {
+                        const T tk = keep(t);
+                        r[m] = jterm ? tk : r[m];
+                    } else {
+                        r[m] = t;
+                    }
                 }
             };
             auto kpart = [&]() {
@@ -533,7 +538,12 @@ struct Band {
                     const T t = __builtin_fmaf(-c.kn[m + 6], nb[2],
                                 __builtin_fmaf(-c.kn[m + 3], nb[1],
                                 __builtin_fmaf(-c.kn[m], nb[0], r[m])));
-                    r[m] = KSEL ? (kterm ? keep(t) : r[m]) : t;
+                    if constexpr (KSEL) {
+                        const T tk = keep(t);
+                        r[m] = kterm ? tk : r[m];
+                    } else {
+                        r[m] = t;
+                    }
                 }
             };
             if constexpr (FWD) {
@@ -551,7 +561,12 @@ struct Band {
                 for (int m = 0; m < 3; ++m) {
                     const double t = r[m] - c.jn[m] * own[0] - c.jn[m + 3] * own[1] -
                                      c.jn[m + 6] * own[2];
-                    r[m] = JSEL ? (jterm ? keep(t) : r[m]) : t;
+                    if constexpr (JSEL) {
+                        const double tk = keep(t);
+                        r[m] = jterm ? tk : r[m];
+                    } else {
+                        r[m] = t;
+                    }
                 }
             };
             auto kpart = [&]() {
@@ -559,7 +574,12 @@ struct Band {
                 for (int m = 0; m < 3; ++m) {
                     const double t = r[m] - c.kn[m] * nb[0] - c.kn[m + 3] * nb[1] -
                                      c.kn[m + 6] * nb[2];
-                    r[m] = KSEL ? (kterm ? keep(t) : r[m]) : t;
+                    if constexpr (KSEL) {
+                        const double tk = keep(t);
+                        r[m] = kterm ? tk : r[m];
+                    } else {
+                        r[m] = t;
+                    }
                 }
             };
             if constexpr (FWD) {
@@ -574,12 +594,20 @@ struct Band {
             // f*(r - a*x - b*y) with the terms of the reference's branches (f*r == r*f)
             double t = c.r[0];
             auto jpart = [&]() {
-                const double u = keep(t - c.jn[0] * own[0]);
-                t = JSEL ? (jterm ? u : t) : u;
+                if constexpr (JSEL) {
+                    const double u = keep(t - c.jn[0] * own[0]);
+                    t = jterm ? u : t;
+                } else {
+                    t = t - c.jn[0] * own[0];
+                }
             };
             auto kpart = [&]() {
-                const double u = keep(t - c.kn[0] * nb[0]);
-                t = KSEL ? (kterm ? u : t) : u;
+                if constexpr (KSEL) {
+                    const double u = keep(t - c.kn[0] * nb[0]);
+                    t = kterm ? u : t;
+                } else {
+                    t = t - c.kn[0] * nb[0];
+                }
             };
             if constexpr (FWD) {
                 jpart();
@@ -696,7 +724,7 @@ k_band(Dims d, LusgsWs w)
 
 void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &w)
 {
-    if (w.pf3 == 0) w.pf3 = 3;
+    if (w.pf3 == 0) w.pf3 = 4;
     if (w.pf1 == 0) w.pf1 = 8;
     w.rows = rows < 1 ? 1 : (rows > BAND_LANES ? BAND_LANES : rows);
     w.nband = (km + w.rows - 1) / w.rows;
