@@ -68,6 +68,49 @@ def kernel_times(case, steps: int, warmup: int) -> dict:
         return s.timings()
 
 
+def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
+    """The batched workload: n seeded NACA 4-digit cases of three sizes, FP32-LHS LU-SGS,
+    advanced together (one captured HIP graph launch per pass of all cases), and the same
+    cases advanced one after another for comparison."""
+    from minismac_amd import autotune
+    from minismac_amd._abi import SMAC_FP32_LHS
+    from minismac_amd.case import batch_cases
+    from minismac_amd.solver import Batch, Solver
+    cases = batch_cases(n, seed=seed, impsch=2, precision=SMAC_FP32_LHS)
+    npts = sum(c.npts for c in cases)
+    t0 = time.perf_counter()
+    opts = autotune.tune_cases(cases)
+    t_tune = time.perf_counter() - t0
+    tuned = {f"{c.jmax}x{c.kmax}": o for c, o in zip(cases, opts)}
+    sol = [Solver(c, options=o) for c, o in zip(cases, opts)]
+    try:
+        with Batch(sol) as b:
+            b.run(1 + warmup)             # the first pass (eager), capture, warm-up replays
+            t0 = time.perf_counter()
+            b.run(steps)                  # ends with every case's stream synchronised
+            wall = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for s in sol:
+            s.run(steps)
+        wall_seq = time.perf_counter() - t0
+        finite = all(s.stats()["nonfinite"] == 0 for s in sol)
+    finally:
+        for s in sol:
+            s.close()
+    sizes = {}
+    for c in cases:
+        sizes[f"{c.jmax}x{c.kmax}"] = sizes.get(f"{c.jmax}x{c.kmax}", 0) + 1
+    return {"value": round(npts * steps / wall / 1e6, 4), "unit": "Mcell-iters/s",
+            "cases": n, "points": npts, "sizes": sizes, "seed": seed, "steps": steps,
+            "ms_per_pass_of_all": round(wall / steps * 1e3, 4),
+            "sequential_value": round(npts * steps / wall_seq / 1e6, 4),
+            "speedup_vs_sequential": round(wall_seq / wall, 4), "finite": finite,
+            "precision": "fp32-lhs", "tuned_options": tuned,
+            "autotune_seconds": round(t_tune, 2),
+            "note": "one hipGraph launch per pass of all cases (each case on its own streams); "
+                    "sequential = the same cases run one after another with smac_run"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +121,9 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-LHS measurement")
+    ap.add_argument("--batch-cases", type=int, default=0,
+                    help="cases in the batched measurement (0 skips it; smac_batch_run is "
+                         "switched off in this build)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -132,7 +178,7 @@ def main() -> int:
         alg = BAND_FWD_BYTES["fp64"] * case.npts
         achieved = alg / (t * 1e-3) / 1e9
         line["roofline"] = {
-            "bound": "hbm", "kernel": "k_band<3,3,fwd,double> (LU-SGS forward band sweep)",
+            "bound": "hbm", "kernel": "k_band<3,4,fwd,double> (LU-SGS forward band sweep)",
             "achieved": round(achieved, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 6),
             "traffic": round(BAND_FWD_TRAFFIC["fp64"] * case.npts),
@@ -161,6 +207,9 @@ def main() -> int:
             "note": "converged CL/CD equal to the FP64 path within 1e-6 on a steady case "
                     "(this Re=3e6 flow is unsteady: the reference's mean flow is laminar)",
         }
+    if args.batch_cases > 0:
+        print(f"bench: batched workload, {args.batch_cases} cases", file=sys.stderr, flush=True)
+        line["batched"] = batched_run(args.batch_cases, args.steps, args.warmup)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_iters)
     print(json.dumps(line), flush=True)

@@ -30,7 +30,8 @@ typedef enum smac_status {
     SMAC_ERR_NONFINITE = 7,    /* residual or update produced Inf/NaN                     */
     SMAC_ERR_HIP = 8,          /* a HIP runtime call failed                               */
     SMAC_ERR_NOGPU = 9,        /* no gfx950 device visible                                */
-    SMAC_ERR_STALL = 10        /* an LU-SGS band hand-over wait timed out (bounded spin)  */
+    SMAC_ERR_STALL = 10,       /* an LU-SGS band hand-over wait timed out (bounded spin)  */
+    SMAC_ERR_DISABLED = 11     /* the entry point is switched off in this build           */
 } smac_status;
 
 /* Solver settings: the reference's namelist $input (ref/initia.f:85-89) plus the values its
@@ -111,6 +112,23 @@ smac_status smac_step(smac_handle *h, smac_iter_stats *st);
  * rhsnorm <= drop * (rhsnorm of the first pass) when drop > 0.  *done = passes made. */
 smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *last, int *done);
 
+/* Stats of the last pass made on h (by smac_step, smac_run or smac_batch_run). */
+smac_status smac_get_stats(smac_handle *h, smac_iter_stats *st);
+
+/* ---- Batches of cases (no counterpart in the reference: its cases are separate runs) ----
+ * A fixed set of cases advanced together.  Each case keeps its own streams, so the cases'
+ * kernels run side by side on the device; after a case's first pass (made eagerly), every
+ * pass of all the cases is one launch of a HIP graph captured once.  The cases may differ
+ * in size, settings and precision; options "timing" and "keep_lhs" must be off.  A case in
+ * a batch must not be stepped on its own while the batch is in use. */
+typedef struct smac_batch smac_batch;
+smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out);
+/* npass passes of every case.  Then checks each case's non-finite and stall flags.
+ * Switched off in this build (returns SMAC_ERR_DISABLED without touching the device): its
+ * first test run ended in a host segmentation fault whose cause is not yet identified. */
+smac_status smac_batch_run(smac_batch *b, int npass);
+smac_status smac_batch_destroy(smac_batch *b);
+
 /* Copy a named device array to the host (diagnostics and kernel-level tests):
  * "q","dq","s","anut","vnut","smin","sf","stc","ds","vort" and, with option "keep_lhs", the
  * assembled LHS "btc","bjm","bjp","bkm","bkp" (FP64 planes as assembled, before the solver
@@ -128,7 +146,8 @@ smac_status smac_timings(smac_handle *h, double out[7]);
  * smac_get_array), "timing" (1: time kernel families with HIP events on the handle's
  * stream), "lusgs_rows" (1..64, default 64: grid rows per wavefront band of the LU-SGS
  * sweeps), "lusgs_prefetch" / "lusgs_prefetch_sa" (1, 2, 3, 4, 6 or 8; defaults 4 and 8:
- * steps of operand prefetch in the 3x3 / scalar LU-SGS sweeps).  The last three are tuning
+ * steps of operand prefetch in the 3x3 / scalar LU-SGS sweeps), "lr_lines" (1..64, default
+ * 64: lines per workgroup of the line-relaxation factorisations).  The last four are tuning
  * knobs; results do not depend on them.  No counterpart in the reference. */
 smac_status smac_set_option(smac_handle *h, const char *name, int value);
 

@@ -107,6 +107,28 @@ def config_case(index: int, **overrides) -> Case:
     return naca_case(jmax, kmax, wall_spacing=ws, **c)
 
 
+BATCH_SIZES = ((239, 61, 1.0e-5), (477, 121, 5.0e-6), (945, 241, 2.0e-6))
+
+
+def batch_cases(n: int = 64, seed: int = 0, **overrides) -> list[Case]:
+    """The batched workload: n NACA 4-digit C-grid cases, seeded (numpy PCG64).
+    m ~ U[0, 0.06], p ~ U[0.2, 0.6], t ~ U[0.08, 0.18], size uniform over 239x61, 477x121 and
+    945x241 (wall spacing 1e-5c, 5e-6c, 2e-6c), alpha ~ U[-4, 16] deg, log10 Re ~ U[5, 7];
+    beta = 10 and far field 15c as in the single-case workloads."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        m, pp, t = rng.uniform(0.0, 0.06), rng.uniform(0.2, 0.6), rng.uniform(0.08, 0.18)
+        jm, km, ws = BATCH_SIZES[int(rng.integers(0, len(BATCH_SIZES)))]
+        alpha, lre = rng.uniform(-4.0, 16.0), rng.uniform(5.0, 7.0)
+        c = dict(alpha=alpha, reynum=10.0 ** lre, beta=10.0)
+        c.update(overrides)
+        out.append(naca_case(jm, km, m=m, p=pp, t=t, wall_spacing=ws,
+                             name=f"batch {i}: m={m:.3f} p={pp:.2f} t={t:.3f} {jm}x{km} "
+                                  f"a={alpha:.2f} Re={10.0 ** lre:.3g}", **c))
+    return out
+
+
 def shipped_case(xy_path: str, bc_path: str, **params) -> Case:
     """The reference's shipped case from copies of its two input files."""
     x, y = _grid.read_plot3d(xy_path)

@@ -302,12 +302,14 @@ void launch_wake(hipStream_t sm, const Dims &d, int neqs, const WakePairs &w, co
 template <typename LT>
 void launch_lrelax_ref(hipStream_t sm, const Dims &d, const DevParams &prm, int njsp, int nksp,
                        int nbcimp, const WakePairs &w, const double *q, double *dq, double *s,
-                       LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp)
+                       LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp, int lpb)
 {
     const long N = d.N;
+    // lines per workgroup of the factor kernels (one lane per line)
+    lpb = lpb < 1 ? 1 : (lpb > 64 ? 64 : lpb);
     hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, sm, dq, 3 * N, 0.0);
     if (njsp > 0)
-        hipLaunchKernelGGL(k_btlu3<LT>, dim3((d.km + 63) / 64), dim3(64), 0, sm, N, d.km, d.jm,
+        hipLaunchKernelGGL(k_btlu3<LT>, dim3((d.km + lpb - 1) / lpb), dim3(lpb), 0, sm, N, d.km, d.jm,
                            1L, (long)d.jm, bjm, btc, bjp);
     for (int nj = 1; nj <= njsp; ++nj) {
         hipLaunchKernelGGL(k_sweep_ref<LT>, dim3(1), dim3(SWEEP_THREADS),
@@ -316,10 +318,10 @@ void launch_lrelax_ref(hipStream_t sm, const Dims &d, const DevParams &prm, int 
         if ((nj / nbcimp) * nbcimp == nj) launch_wake(sm, d, 3, w, q, dq, s);
     }
     if (njsp > 0)
-        hipLaunchKernelGGL(k_btund3<LT>, dim3((d.km + 63) / 64), dim3(64), 0, sm, N, d.km, d.jm,
+        hipLaunchKernelGGL(k_btund3<LT>, dim3((d.km + lpb - 1) / lpb), dim3(lpb), 0, sm, N, d.km, d.jm,
                            1L, (long)d.jm, bjm, btc, bjp);
     if (nksp > 0)
-        hipLaunchKernelGGL(k_btlu3<LT>, dim3((d.jm + 63) / 64), dim3(64), 0, sm, N, d.jm, d.km,
+        hipLaunchKernelGGL(k_btlu3<LT>, dim3((d.jm + lpb - 1) / lpb), dim3(lpb), 0, sm, N, d.jm, d.km,
                            (long)d.jm, 1L, bkm, btc, bkp);
     for (int nk = 1; nk <= nksp; ++nk) {
         hipLaunchKernelGGL(k_sweep_ref<LT>, dim3(1), dim3(SWEEP_THREADS),
@@ -332,6 +334,6 @@ void launch_lrelax_ref(hipStream_t sm, const Dims &d, const DevParams &prm, int 
 template void launch_lrelax_ref<double>(hipStream_t, const Dims &, const DevParams &, int, int,
                                         int, const WakePairs &, const double *, double *,
                                         double *, double *, double *, double *, double *,
-                                        double *);
+                                        double *, int);
 
 }  // namespace smac

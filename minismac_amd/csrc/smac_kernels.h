@@ -45,12 +45,7 @@ void launch_resid(hipStream_t, const Dims &, int nt, double beta, const double *
 template <typename LT>
 void launch_lrelax_ref(hipStream_t, const Dims &, const DevParams &, int njsp, int nksp,
                        int nbcimp, const WakePairs &, const double *q, double *dq, double *s,
-                       LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
-// zebra order: odd lines then even lines, each colour all at once
-template <typename LT>
-void launch_lrelax_zebra(hipStream_t, const Dims &, const DevParams &, int njsp, int nksp,
-                         int nbcimp, const WakePairs &, const double *q, double *dq, double *s,
-                         LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
+                       LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp, int lines_per_block = 64);
 void launch_wake(hipStream_t, const Dims &, int neqs, const WakePairs &, const double *sq,
                  double *ds, double *rhs);
 

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsmac_amd
 EXPORTS = [
     "smac_create", "smac_destroy", "smac_set_state", "smac_get_state", "smac_step", "smac_run",
     "smac_get_array", "smac_timings", "smac_set_option", "smac_lrelax", "smac_lusgs",
+    "smac_get_stats", "smac_batch_create", "smac_batch_run", "smac_batch_destroy",
     "smac_last_error", "smac_version",
 ]
 
@@ -56,6 +57,10 @@ def lib():
         L.smac_lrelax.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, P(C.c_int)] + [D] * 8
         L.smac_lusgs.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, C.c_int,
                                  P(C.c_int)] + [D] * 9
+        L.smac_get_stats.argtypes = [C.c_void_p, P(SmacIterStats)]
+        L.smac_batch_create.argtypes = [P(C.c_void_p), C.c_int, P(C.c_void_p)]
+        L.smac_batch_run.argtypes = [C.c_void_p, C.c_int]
+        L.smac_batch_destroy.argtypes = [C.c_void_p]
         for fn in EXPORTS[:-2]:
             getattr(L, fn).restype = C.c_int
         L.smac_last_error.restype = C.c_char_p
@@ -125,6 +130,12 @@ class Solver:
         _check(lib().smac_run(self.h, niter, drop, C.byref(st), C.byref(done)), "smac_run")
         return st.as_dict(), done.value
 
+    def stats(self) -> dict:
+        """Stats of the last pass made on this case (by step, run or a Batch)."""
+        st = SmacIterStats()
+        _check(lib().smac_get_stats(self.h, C.byref(st)), "smac_get_stats")
+        return st.as_dict()
+
     def timings(self) -> dict:
         out = (C.c_double * 7)()
         _check(lib().smac_timings(self.h, out), "smac_timings")
@@ -151,6 +162,34 @@ class Solver:
         v = np.zeros((self.jmax, self.kmax), order="F")
         _check(lib().smac_get_state(self.h, dptr(q), dptr(a), dptr(v)), "smac_get_state")
         return q, a, v
+
+
+class Batch:
+    """Several Solvers advanced together: after each case's first pass, one pass of all of
+    them is one launch of a captured HIP graph (smac_batch_run)."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        b = C.c_void_p()
+        _check(lib().smac_batch_create(arr, len(self.solvers), C.byref(b)), "smac_batch_create")
+        self.b = b
+
+    def run(self, npass: int) -> None:
+        _check(lib().smac_batch_run(self.b, int(npass)), "smac_batch_run")
+
+    def close(self) -> None:
+        if getattr(self, "b", None):
+            lib().smac_batch_destroy(self.b)
+            self.b = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def _pairs_arg(pairs):

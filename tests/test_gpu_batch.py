@@ -1,0 +1,128 @@
+"""Batches: several cases advanced together, each pass of all of them one launch of a captured
+HIP graph (smac_batch_run).  A batched case must follow exactly the trajectory the same case
+follows when it is stepped on its own (bit for bit: the graph replays the same kernels on the
+same buffers), across several run calls with host reads and state changes between them.
+"""
+import numpy as np
+import pytest
+
+from minismac_amd._abi import SMAC_FP32_LHS
+from minismac_amd.case import batch_cases, config_case, naca_case
+from minismac_amd.solver import Batch, SmacError, Solver
+
+pytestmark = pytest.mark.gpu
+
+# smac_batch_run is switched off (SMAC_ERR_DISABLED): its first run, the first test below,
+# ended in a host segmentation fault inside smac_batch_run whose cause is not identified.
+# The tests that run batches stay skipped until it is.
+BATCH_OFF = pytest.mark.skip(reason="smac_batch_run switched off: unexplained host "
+                                    "segmentation fault in its first run")
+
+
+def _mixed_cases():
+    return [
+        naca_case(239, 61, m=0.02, p=0.3, t=0.10, alpha=6.0, reynum=2.0e5, beta=10.0, impsch=2),
+        config_case(0, impsch=1),
+        naca_case(239, 61, m=0.05, p=0.5, t=0.15, alpha=-2.0, reynum=5.0e6, beta=10.0,
+                  impsch=2, precision=SMAC_FP32_LHS),
+        naca_case(477, 121, m=0.0, p=0.4, t=0.09, alpha=12.0, reynum=1.0e6, beta=10.0, impsch=2),
+    ]
+
+
+def _same(a: Solver, b: Solver, tag: str):
+    qa, aa, va = a.get_state()
+    qb, ab, vb = b.get_state()
+    assert np.array_equal(qa, qb), f"{tag}: q differs ({a.case.name})"
+    assert np.array_equal(aa, ab), f"{tag}: nu~ differs ({a.case.name})"
+    assert np.array_equal(va, vb), f"{tag}: nu_t differs ({a.case.name})"
+    sa, sb = a.stats(), b.stats()
+    for k in ("nt", "rhsnorm", "resmax", "cl", "cd", "cm"):
+        assert sa[k] == sb[k], f"{tag}: {k} {sa[k]!r} vs {sb[k]!r} ({a.case.name})"
+
+
+@BATCH_OFF
+def test_batch_graph_replays_match_eager_runs():
+    cases = _mixed_cases()
+    eager = [Solver(c) for c in cases]
+    batched = [Solver(c) for c in cases]
+    try:
+        with Batch(batched) as b:
+            for call, npass in enumerate((1, 3, 2, 4)):   # call 0 is the eager first pass
+                print(f"batch call {call}: {npass} passes of {len(cases)} cases", flush=True)
+                b.run(npass)
+                for s in eager:
+                    s.run(npass)
+                for se, sb in zip(eager, batched):
+                    _same(se, sb, f"call {call}")
+            # host-side state changes between replays reach the graph's buffers
+            for se, sb in zip(eager, batched):
+                q, a, _ = se.get_state()
+                q = q * (1.0 + 1e-3)
+                st = se.stats()
+                for s in (se, sb):
+                    s.set_state(q=q, anut=a, nt=st["nt"], resmax0=1.0, clift=st["cl"])
+            b.run(3)
+            for s in eager:
+                s.run(3)
+            for se, sb in zip(eager, batched):
+                _same(se, sb, "after set_state")
+    finally:
+        for s in eager + batched:
+            s.close()
+
+
+@BATCH_OFF
+def test_batch_first_passes_after_reset_run_eagerly():
+    """Resetting a case to nt=0 between calls makes its next pass a first pass again (wall
+    distance, resmax0 latch) -- made eagerly, then the captured graph takes over."""
+    cases = _mixed_cases()[:2]
+    eager = [Solver(c) for c in cases]
+    batched = [Solver(c) for c in cases]
+    try:
+        with Batch(batched) as b:
+            b.run(3)
+            for s in eager:
+                s.run(3)
+            for s in (eager[0], batched[0]):
+                s.set_state(q=None, anut=None, nt=0, resmax0=1.0, clift=0.0)
+            b.run(3)
+            for s in eager:
+                s.run(3)
+            for se, sb in zip(eager, batched):
+                _same(se, sb, "after reset")
+    finally:
+        for s in eager + batched:
+            s.close()
+
+
+def test_batch_rejects_timing_and_duplicates():
+    c = config_case(0, impsch=2)
+    with Solver(c, timing=True) as t, Solver(c) as s:
+        with pytest.raises(SmacError):
+            Batch([t, s])
+        with pytest.raises(SmacError):
+            Batch([s, s])
+        # while switched off, a run reports so and touches nothing
+        with Batch([s]) as b:
+            with pytest.raises(SmacError) as e:
+                b.run(1)
+            assert e.value.status == 11
+        assert s.stats()["nt"] == 0
+
+
+@BATCH_OFF
+def test_batch_workload_generator_cases_step():
+    """A few of the batched workload's generated cases (seeded) take finite passes."""
+    cs = batch_cases(64, seed=0)
+    assert len({(c.jmax, c.kmax) for c in cs}) == 3
+    pick = [c for c in cs if c.jmax == 239][:2] + [c for c in cs if c.jmax == 477][:1]
+    sol = [Solver(c) for c in pick]
+    try:
+        with Batch(sol) as b:
+            b.run(5)
+        for s in sol:
+            st = s.stats()
+            assert st["nt"] == 5 and np.isfinite(st["rhsnorm"]) and np.isfinite(st["cl"])
+    finally:
+        for s in sol:
+            s.close()
