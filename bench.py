@@ -70,8 +70,8 @@ def kernel_times(case, steps: int, warmup: int) -> dict:
 
 def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
     """The batched workload: n seeded NACA 4-digit cases of three sizes, FP32-LHS LU-SGS,
-    advanced together (one captured HIP graph launch per pass of all cases), and the same
-    cases advanced one after another for comparison."""
+    advanced together (smac_batch_run), and the same cases advanced one after another for
+    comparison."""
     from minismac_amd import autotune
     from minismac_amd._abi import SMAC_FP32_LHS
     from minismac_amd.case import batch_cases
@@ -85,7 +85,7 @@ def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
     sol = [Solver(c, options=o) for c, o in zip(cases, opts)]
     try:
         with Batch(sol) as b:
-            b.run(1 + warmup)             # the first pass (eager), capture, warm-up replays
+            b.run(1 + warmup)             # the first pass (wall distance) and warm-up
             t0 = time.perf_counter()
             b.run(steps)                  # ends with every case's stream synchronised
             wall = time.perf_counter() - t0
@@ -107,8 +107,8 @@ def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
             "speedup_vs_sequential": round(wall_seq / wall, 4), "finite": finite,
             "precision": "fp32-lhs", "tuned_options": tuned,
             "autotune_seconds": round(t_tune, 2),
-            "note": "one hipGraph launch per pass of all cases (each case on its own streams); "
-                    "sequential = the same cases run one after another with smac_run"}
+            "note": "one stream; the LU-SGS band sweeps of all cases in one launch per "
+                    "direction; sequential = the same cases run one after another with smac_run"}
 
 
 def main() -> int:
@@ -121,9 +121,8 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-LHS measurement")
-    ap.add_argument("--batch-cases", type=int, default=0,
-                    help="cases in the batched measurement (0 skips it; smac_batch_run is "
-                         "switched off in this build)")
+    ap.add_argument("--batch-cases", type=int, default=64,
+                    help="cases in the batched measurement (0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device (scale for HBM byte counters)")
     args = ap.parse_args()

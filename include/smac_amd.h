@@ -30,8 +30,7 @@ typedef enum smac_status {
     SMAC_ERR_NONFINITE = 7,    /* residual or update produced Inf/NaN                     */
     SMAC_ERR_HIP = 8,          /* a HIP runtime call failed                               */
     SMAC_ERR_NOGPU = 9,        /* no gfx950 device visible                                */
-    SMAC_ERR_STALL = 10,       /* an LU-SGS band hand-over wait timed out (bounded spin)  */
-    SMAC_ERR_DISABLED = 11     /* the entry point is switched off in this build           */
+    SMAC_ERR_STALL = 10        /* an LU-SGS band hand-over wait timed out (bounded spin)  */
 } smac_status;
 
 /* Solver settings: the reference's namelist $input (ref/initia.f:85-89) plus the values its
@@ -116,16 +115,18 @@ smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *la
 smac_status smac_get_stats(smac_handle *h, smac_iter_stats *st);
 
 /* ---- Batches of cases (no counterpart in the reference: its cases are separate runs) ----
- * A fixed set of cases advanced together.  Each case keeps its own streams, so the cases'
- * kernels run side by side on the device; after a case's first pass (made eagerly), every
- * pass of all the cases is one launch of a HIP graph captured once.  The cases may differ
- * in size, settings and precision; options "timing" and "keep_lhs" must be off.  A case in
- * a batch must not be stepped on its own while the batch is in use. */
+ * A fixed set of cases advanced together on one stream of their own.  The LU-SGS band
+ * sweeps -- a few wavefronts per case, bound by the latency of their hyperplane steps -- run
+ * for all cases in one launch per sweep direction (one for the mean flow of each precision,
+ * one for the SA model), so the bands of every case sweep side by side; the fully parallel
+ * kernels of each case are enqueued between them.  Each case gets exactly the kernels and
+ * operands of its own pass: its results are those of smac_step on it alone.  The cases may
+ * differ in size, scheme, settings and precision; options "timing" and "keep_lhs" must be
+ * off.  A case must not be stepped on its own while a smac_batch_run on it is in progress. */
 typedef struct smac_batch smac_batch;
 smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out);
-/* npass passes of every case.  Then checks each case's non-finite and stall flags.
- * Switched off in this build (returns SMAC_ERR_DISABLED without touching the device): its
- * first test run ended in a host segmentation fault whose cause is not yet identified. */
+/* npass passes of every case; returns after the device has finished them, having checked
+ * each case's non-finite and stall flags. */
 smac_status smac_batch_run(smac_batch *b, int npass);
 smac_status smac_batch_destroy(smac_batch *b);
 

@@ -622,59 +622,216 @@ smac_status smac_get_array(smac_handle *h, const char *name, double *out)
 }  // extern "C"
 
 // ---- batches ------------------------------------------------------------------------------
+//
+// A batch advances its cases together on one stream.  The LU-SGS band sweeps, the part of a
+// pass that occupies only a few wavefronts per case and is bound by the latency of its
+// hyperplane steps, run for every case of the batch in one launch (k_band_multi), so that the
+// bands of all cases sweep side by side on the device; the fully parallel kernels of each case
+// are enqueued in between.  Every case sees exactly the kernels and operands of its own pass,
+// so its results are those of smac_step on it alone.
+
+struct BandGroup {
+    std::vector<int> members;        // indices into cases
+    BandJob *jobs = nullptr;         // device
+    int2 *map = nullptr;             // device
+    int nblk = 0, pf = 3;
+};
 
 struct smac_batch {
     std::vector<smac_handle *> cases;
-    hipStream_t root = nullptr;
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> join;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    BandGroup sa;                    // SA LU-SGS (neqs 1, FP64) of the LU-SGS cases
+    BandGroup mean[2];               // mean-flow LU-SGS: FP64, FP32-LHS
+    std::vector<LusgsWs> snap_mean, snap_sa;   // workspaces the tables were built from
 };
+
+#define B_TRY(x)                   \
+    do {                           \
+        smac_status r_ = (x);      \
+        if (r_) return r_;         \
+    } while (0)
+
+static bool is_lusgs(const smac_handle *h) { return h->p.impsch == 2; }
+static bool has_sa(const smac_handle *h) { return h->dp.ivis == 1 && h->dp.iturb == 1; }
+
+// frees the device tables (the member list stays)
+static void free_group(BandGroup &g)
+{
+    if (g.jobs) (void)hipFree(g.jobs);
+    if (g.map) (void)hipFree(g.map);
+    g.jobs = nullptr;
+    g.map = nullptr;
+    g.nblk = 0;
+}
 
 static void free_batch(smac_batch *b)
 {
     if (!b) return;
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (hipEvent_t e : b->join)
-        if (e) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    if (b->root) (void)hipStreamDestroy(b->root);
+    free_group(b->sa);
+    free_group(b->mean[0]);
+    free_group(b->mean[1]);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
-// Records one pass of every case into b->graph: the root stream forks to each case's stream
-// (and, inside the pass, to its SA stream) and joins them back.
-static smac_status capture_batch(smac_batch *b)
+static bool same_ws(const LusgsWs &a, const LusgsWs &c)
 {
-    std::vector<int> nt0;
-    for (smac_handle *h : b->cases) nt0.push_back(h->nt);
-    HIP_TRY(hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal));
-    smac_status rc = SMAC_OK;
-    if (hipEventRecord(b->fork, b->root) != hipSuccess) rc = fail(SMAC_ERR_HIP, "fork record");
-    for (size_t i = 0; i < b->cases.size() && !rc; ++i) {
+    return a.lhs == c.lhs && a.r == c.r && a.dq == c.dq && a.edge == c.edge &&
+           a.nband == c.nband && a.rows == c.rows && a.pf3 == c.pf3 && a.pf1 == c.pf1;
+}
+
+static smac_status build_group(BandGroup &g, const std::vector<BandJob> &jobs, int pf)
+{
+    free_group(g);
+    if (g.members.size() != jobs.size()) return fail(SMAC_ERR_ARG, "batch table mismatch");
+    std::vector<int2> map;
+    for (size_t i = 0; i < jobs.size(); ++i)
+        for (int blk = 0; blk < jobs[i].w.nband; ++blk) map.push_back(make_int2((int)i, blk));
+    g.nblk = (int)map.size();
+    g.pf = pf;
+    if (jobs.empty()) return SMAC_OK;
+    if (hipMalloc(&g.jobs, jobs.size() * sizeof(BandJob)) != hipSuccess ||
+        hipMalloc(&g.map, map.size() * sizeof(int2)) != hipSuccess)
+        return fail(SMAC_ERR_HIP, "hipMalloc failed (batch band tables)");
+    HIP_TRY(hipMemcpy(g.jobs, jobs.data(), jobs.size() * sizeof(BandJob), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g.map, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice));
+    return SMAC_OK;
+}
+
+// (Re)builds the multi-case band tables from the cases' current workspaces.
+static smac_status build_tables(smac_batch *b)
+{
+    std::vector<BandJob> jsa, jm[2];
+    b->sa.members.clear();
+    b->mean[0].members.clear();
+    b->mean[1].members.clear();
+    b->snap_mean.clear();
+    b->snap_sa.clear();
+    int pf_sa = 0, pf_m[2] = {0, 0};
+    for (size_t i = 0; i < b->cases.size(); ++i) {
         smac_handle *h = b->cases[i];
-        if (hipStreamWaitEvent(h->stream, b->fork, 0) != hipSuccess) {
-            rc = fail(SMAC_ERR_HIP, "fork wait");
-            break;
+        b->snap_mean.push_back(h->lws);
+        b->snap_sa.push_back(h->lws_sa);
+        if (!is_lusgs(h)) continue;
+        LusgsWs w = h->lws;
+        w.ev_fwd[0] = w.ev_fwd[1] = nullptr;
+        const int g = h->p.precision == SMAC_FP32_LHS ? 1 : 0;
+        b->mean[g].members.push_back((int)i);
+        jm[g].push_back(BandJob{h->d, w});
+        if (!pf_m[g]) pf_m[g] = w.pf3;
+        if (has_sa(h)) {
+            LusgsWs ws = h->lws_sa;
+            ws.ev_fwd[0] = ws.ev_fwd[1] = nullptr;
+            b->sa.members.push_back((int)i);
+            jsa.push_back(BandJob{h->d, ws});
+            if (!pf_sa) pf_sa = ws.pf1;
         }
-        rc = enqueue_pass(h);
-        if (!rc && (hipEventRecord(b->join[i], h->stream) != hipSuccess ||
-                    hipStreamWaitEvent(b->root, b->join[i], 0) != hipSuccess))
-            rc = fail(SMAC_ERR_HIP, "join");
     }
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->root, &g);
-    // capturing recorded the pass; it ran nothing: restore the pass counters
-    for (size_t i = 0; i < b->cases.size(); ++i) b->cases[i]->nt = nt0[i];
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+    // the prefetch distance of a multi-case launch is that of its first case (any value
+    // gives the same results)
+    B_TRY(build_group(b->sa, jsa, pf_sa ? pf_sa : 8));
+    B_TRY(build_group(b->mean[0], jm[0], pf_m[0] ? pf_m[0] : 4));
+    B_TRY(build_group(b->mean[1], jm[1], pf_m[1] ? pf_m[1] : 4));
+    return SMAC_OK;
+}
+
+static smac_status check_tables(smac_batch *b)
+{
+    for (size_t i = 0; i < b->cases.size(); ++i)
+        if (!same_ws(b->snap_mean[i], b->cases[i]->lws) ||
+            !same_ws(b->snap_sa[i], b->cases[i]->lws_sa)) {
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            return build_tables(b);
+        }
+    return SMAC_OK;
+}
+
+// one pass of every case, in the order of enqueue_pass for each case
+static smac_status enqueue_batch_pass(smac_batch *b)
+{
+    hipStream_t sm = b->stream;
+    // SA up to its band sweeps (line-relaxation cases: the whole SA step)
+    for (smac_handle *h : b->cases) {
+        h->nt += 1;
+        if (!has_sa(h)) continue;
+        const Dims &d = h->d;
+        SaArrays &a = h->sa;
+        if (h->first_sa) launch_sa_first(sm, d, h->walls, h->x, h->y, a);
+        launch_sa_assemble(sm, d, h->dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
+                           (int)h->regs.size(), h->regs.data(), a);
+        h->first_sa = false;
+        if (is_lusgs(h)) {
+            lusgs_pre<double, 1>(sm, d, h->lws_sa, a.sf, a.stc, a.sjm, a.sjp, a.skm, a.skp, nullptr);
+        } else {
+            launch_sa_solve(sm, d, h->dp, h->p.ntjsp, h->wake, &h->lws_sa, a);
+            launch_sa_update(sm, d, h->dp, h->w, h->st, a);
+        }
     }
-    if (e != hipSuccess) return fail(SMAC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    b->graph = g;
-    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    BandGroup &gs = b->sa;
+    lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true);
+    for (int i : gs.members) lusgs_mid<double, 1>(sm, b->cases[i]->d, b->cases[i]->lws_sa);
+    lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false);
+    for (int i : gs.members) {
+        smac_handle *h = b->cases[i];
+        SaArrays &a = h->sa;
+        lusgs_post<double, 1>(sm, h->d, h->lws_sa, h->wake, a.anut, a.ds, a.sf, false);
+        launch_sa_update(sm, h->d, h->dp, h->w, h->st, a);
+    }
+    // mean flow up to its band sweeps (line-relaxation cases: the whole solve)
+    for (smac_handle *h : b->cases) {
+        const Dims &d = h->d;
+        const DevParams &dp = h->dp;
+        double *btc = h->blk[0], *bjm = h->blk[1], *bjp = h->blk[2], *bkm = h->blk[3],
+               *bkp = h->blk[4];
+        const bool f32 = h->p.precision == SMAC_FP32_LHS;
+        float *fb[5] = {(float *)btc, (float *)bjm, (float *)bjp, (float *)bkm, (float *)bkp};
+        if (f32) {
+            launch_assemble<float>(sm, d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2],
+                                   fb[3], fb[4]);
+            for (const BcRegion &r : h->regs)
+                launch_bc<float>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, fb[0],
+                                 fb[1], fb[2], fb[3], fb[4]);
+        } else {
+            launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm,
+                                    bkp);
+            for (const BcRegion &r : h->regs)
+                launch_bc<double>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm,
+                                  bjp, bkm, bkp);
+        }
+        launch_resid(sm, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
+        if (!is_lusgs(h))
+            launch_lrelax_ref<double>(sm, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake,
+                                      h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lr_lines);
+        else if (f32)
+            lusgs_pre<float, 3>(sm, d, h->lws, h->s, fb[0], fb[1], fb[2], fb[3], fb[4], nullptr);
+        else
+            lusgs_pre<double, 3>(sm, d, h->lws, h->s, btc, bjm, bjp, bkm, bkp, nullptr);
+    }
+    for (int g = 0; g < 2; ++g) {
+        BandGroup &gm = b->mean[g];
+        if (!gm.nblk) continue;
+        if (g == 0) {
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+            for (int i : gm.members) lusgs_mid<double, 3>(sm, b->cases[i]->d, b->cases[i]->lws);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+        } else {
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+            for (int i : gm.members) lusgs_mid<float, 3>(sm, b->cases[i]->d, b->cases[i]->lws);
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+        }
+    }
+    for (smac_handle *h : b->cases) {
+        const Dims &d = h->d;
+        if (is_lusgs(h)) {
+            if (h->p.precision == SMAC_FP32_LHS)
+                lusgs_post<float, 3>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, false);
+            else
+                lusgs_post<double, 3>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, false);
+        }
+        launch_update_force(sm, d, h->dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq,
+                            h->w, h->st);
+    }
+    HIP_TRY(hipGetLastError());
     return SMAC_OK;
 }
 
@@ -700,66 +857,41 @@ smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out)
             }
         b->cases.push_back(h);
     }
-    b->join.assign(n, nullptr);
-    if (hipStreamCreate(&b->root) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
         free_batch(b);
-        return fail(SMAC_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
+        return fail(SMAC_ERR_HIP, "hipStreamCreate failed");
     }
-    for (auto &e : b->join)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            free_batch(b);
-            return fail(SMAC_ERR_HIP, "hipEventCreate failed");
-        }
+    smac_status rc = build_tables(b);
+    if (rc) {
+        free_batch(b);
+        return rc;
+    }
     *out = b;
     return SMAC_OK;
 }
-
-// Off until the crash in the first test run of this path is explained: that run (an eager
-// first pass of four cases, then capture and replays) ended in a host segmentation fault
-// inside this function, and reading this code has not found the cause.
-static constexpr bool k_batch_enabled = false;
 
 smac_status smac_batch_run(smac_batch *b, int npass)
 {
     if (!b) return fail(SMAC_ERR_ARG, "batch is NULL");
     if (npass < 0) return fail(SMAC_ERR_ARG, "npass < 0");
-    if (!k_batch_enabled)
-        return fail(SMAC_ERR_DISABLED, "smac_batch_run is switched off in this build");
-    for (int it = 0; it < npass; ++it) {
-        bool first = false;
-        for (smac_handle *h : b->cases) first = first || h->nt == 0;
-        if (first) {
-            // a case's first pass computes the wall distance and latches resmax0: eagerly
-            for (smac_handle *h : b->cases) {
-                smac_status rc = enqueue_pass(h);
-                if (rc) return rc;
-            }
-            continue;
-        }
-        if (!b->exec) {
-            smac_status rc = capture_batch(b);
-            if (rc) return rc;
-        }
-        // the graph starts after whatever the cases' own streams hold
-        for (size_t i = 0; i < b->cases.size(); ++i) {
-            HIP_TRY(hipEventRecord(b->join[i], b->cases[i]->stream));
-            HIP_TRY(hipStreamWaitEvent(b->root, b->join[i], 0));
-        }
-        HIP_TRY(hipGraphLaunch(b->exec, b->root));
-        for (smac_handle *h : b->cases) h->nt += 1;
-        // later work on a case's own stream comes after the graph
-        HIP_TRY(hipEventRecord(b->fork, b->root));
-        for (smac_handle *h : b->cases) HIP_TRY(hipStreamWaitEvent(h->stream, b->fork, 0));
+    // the batch's stream starts after whatever the cases' own streams hold
+    for (smac_handle *h : b->cases) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream_sa));
     }
-    HIP_TRY(hipStreamSynchronize(b->root));
+    B_TRY(check_tables(b));
+    for (int it = 0; it < npass; ++it) B_TRY(enqueue_batch_pass(b));
+    HIP_TRY(hipStreamSynchronize(b->stream));
     for (size_t i = 0; i < b->cases.size(); ++i) {
         smac_handle *h = b->cases[i];
-        HIP_TRY(hipStreamSynchronize(h->stream));
         DevStats s;
         HIP_TRY(hipMemcpy(&s, h->st, sizeof s, hipMemcpyDeviceToHost));
-        if (s.stall) return fail(SMAC_ERR_STALL, "batch case " + std::to_string(i) + ": LU-SGS band hand-over timed out");
-        if (s.nonfinite) return fail(SMAC_ERR_NONFINITE, "batch case " + std::to_string(i) + ": non-finite at nt=" + std::to_string(h->nt));
+        if (s.stall)
+            return fail(SMAC_ERR_STALL, "batch case " + std::to_string(i) +
+                                            ": LU-SGS band hand-over timed out");
+        if (s.nonfinite)
+            return fail(SMAC_ERR_NONFINITE, "batch case " + std::to_string(i) +
+                                                ": non-finite at nt=" + std::to_string(h->nt));
     }
     return SMAC_OK;
 }
@@ -767,7 +899,7 @@ smac_status smac_batch_run(smac_batch *b, int npass)
 smac_status smac_batch_destroy(smac_batch *b)
 {
     if (!b) return fail(SMAC_ERR_ARG, "batch is NULL");
-    (void)hipStreamSynchronize(b->root);
+    (void)hipStreamSynchronize(b->stream);
     free_batch(b);
     return SMAC_OK;
 }

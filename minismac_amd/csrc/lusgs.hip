@@ -642,14 +642,14 @@ struct Band {
     }
 };
 
+// One band of one case: blk is the band's place in dispatch order (forward bands bottom up,
+// backward bands top down, so that every band is dispatched after the band it waits for).
 template <int NEQ, int PF, bool FWD, typename T>
-__global__ void __launch_bounds__(BAND_LANES)
-k_band(Dims d, LusgsWs w)
+SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
 {
     constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ, T>();
-    const int lane = threadIdx.x, nband = gridDim.x;
-    // backward bands are numbered top down so that workgroups start in dependency order
-    const int b = FWD ? (int)blockIdx.x : nband - 1 - (int)blockIdx.x;
+    const int lane = threadIdx.x, nband = w.nband;
+    const int b = FWD ? blk : nband - 1 - blk;
     const int k0 = b * w.rows;
     const int klast = (k0 + w.rows < d.km ? k0 + w.rows : d.km) - 1;
     const bool real = lane < w.rows;
@@ -720,6 +720,26 @@ k_band(Dims d, LusgsWs w)
     }
 }
 
+template <int NEQ, int PF, bool FWD, typename T>
+__global__ void __launch_bounds__(BAND_LANES)
+k_band(Dims d, LusgsWs w)
+{
+    band_run<NEQ, PF, FWD, T>(d, w, (int)blockIdx.x);
+}
+
+// The bands of several cases in one launch: workgroup i runs band map[i].y of case map[i].x.
+// The map lists each case's bands contiguously in dispatch order, so every wait is on a
+// workgroup dispatched earlier.
+template <int NEQ, int PF, bool FWD, typename T>
+__global__ void __launch_bounds__(BAND_LANES)
+k_band_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const Dims d = jobs[m.x].d;
+    const LusgsWs w = jobs[m.x].w;
+    band_run<NEQ, PF, FWD, T>(d, w, m.y);
+}
+
 // ------------------------------------------------------------------------------------------
 
 void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &w)
@@ -756,42 +776,89 @@ bool lusgs_ws_fits(const LusgsWs &w)
 }
 
 template <typename LT, int NEQ>
-static void launch_lusgs(hipStream_t sm, const Dims &d, const LusgsWs &w, const WakePairs &wk,
-                         const double *q, double *dq, double *rhs, const LT *btc, const LT *bjm,
-                         const LT *bjp, const LT *bkm, const LT *bkp, LT *fac, bool rhs_out)
+void lusgs_pre(hipStream_t sm, const Dims &d, const LusgsWs &w, const double *rhs, const LT *btc,
+               const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *fac)
 {
     dim3 tg((d.jm + TJ - 1) / TJ, (d.km + TK - 1) / TK);
     hipLaunchKernelGGL((k_shear<LT, NEQ>), tg, dim3(TILE_THREADS), 0, sm, d, w, rhs, btc, bjm,
                        bkm, bjp, bkp, fac);
-    const int pf = NEQ == 3 ? w.pf3 : w.pf1;
-#define BAND_LAUNCH(PFV, FWDV)                                                                 \
-    hipLaunchKernelGGL((k_band<NEQ, PFV, FWDV, LT>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
-#define BAND_SWITCH(FWDV)                                                                      \
+}
+
+#define BAND_SWITCH(PFSEL, LAUNCH)                                                             \
     do {                                                                                       \
-        switch (pf) {                                                                          \
-        case 1: BAND_LAUNCH(1, FWDV); break;                                                   \
-        case 2: BAND_LAUNCH(2, FWDV); break;                                                   \
-        case 4: BAND_LAUNCH(4, FWDV); break;                                                   \
-        case 6: BAND_LAUNCH(6, FWDV); break;                                                   \
-        case 8: BAND_LAUNCH(8, FWDV); break;                                                   \
-        default: BAND_LAUNCH(3, FWDV); break;                                                  \
+        switch (PFSEL) {                                                                       \
+        case 1: LAUNCH(1); break;                                                              \
+        case 2: LAUNCH(2); break;                                                              \
+        case 4: LAUNCH(4); break;                                                              \
+        case 6: LAUNCH(6); break;                                                              \
+        case 8: LAUNCH(8); break;                                                              \
+        default: LAUNCH(3); break;                                                             \
         }                                                                                      \
     } while (0)
-    if (w.ev_fwd[0]) (void)hipEventRecord(w.ev_fwd[0], sm);
-    BAND_SWITCH(true);
-    if (w.ev_fwd[1]) (void)hipEventRecord(w.ev_fwd[1], sm);
+
+template <typename LT, int NEQ>
+void lusgs_band(hipStream_t sm, const Dims &d, const LusgsWs &w, bool fwd)
+{
+    const int pf = NEQ == 3 ? w.pf3 : w.pf1;
+#define ONE_F(PFV) hipLaunchKernelGGL((k_band<NEQ, PFV, true, LT>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
+#define ONE_B(PFV) hipLaunchKernelGGL((k_band<NEQ, PFV, false, LT>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
+    if (fwd)
+        BAND_SWITCH(pf, ONE_F);
+    else
+        BAND_SWITCH(pf, ONE_B);
+#undef ONE_F
+#undef ONE_B
+}
+
+template <typename LT, int NEQ>
+void lusgs_band_multi(hipStream_t sm, const BandJob *jobs, const int2 *map, int nblk, int pf,
+                      bool fwd)
+{
+    if (nblk <= 0) return;
+#define MUL_F(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, true, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map)
+#define MUL_B(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, false, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map)
+    if (fwd)
+        BAND_SWITCH(pf, MUL_F);
+    else
+        BAND_SWITCH(pf, MUL_B);
+#undef MUL_F
+#undef MUL_B
+}
+#undef BAND_SWITCH
+
+template <typename LT, int NEQ>
+void lusgs_mid(hipStream_t sm, const Dims &d, const LusgsWs &w)
+{
     long nb = (w.ns + 255) / 256;
     hipLaunchKernelGGL((k_mid<LT, NEQ>), dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(256), 0, sm,
                        d, w);
-    BAND_SWITCH(false);
-#undef BAND_SWITCH
-#undef BAND_LAUNCH
+}
+
+template <typename LT, int NEQ>
+void lusgs_post(hipStream_t sm, const Dims &d, const LusgsWs &w, const WakePairs &wk,
+                const double *q, double *dq, double *rhs, bool rhs_out)
+{
+    dim3 tg((d.jm + TJ - 1) / TJ, (d.km + TK - 1) / TK);
     hipLaunchKernelGGL((k_unshear<NEQ, LT>), tg, dim3(TILE_THREADS), 0, sm, d, w,
                        (const LT *)w.dq, dq);
     if (rhs_out)
         hipLaunchKernelGGL((k_unshear<NEQ, LT>), tg, dim3(TILE_THREADS), 0, sm, d, w,
                            (const LT *)w.r, rhs);
     launch_wake(sm, d, NEQ, wk, q, dq, rhs);
+}
+
+template <typename LT, int NEQ>
+static void launch_lusgs(hipStream_t sm, const Dims &d, const LusgsWs &w, const WakePairs &wk,
+                         const double *q, double *dq, double *rhs, const LT *btc, const LT *bjm,
+                         const LT *bjp, const LT *bkm, const LT *bkp, LT *fac, bool rhs_out)
+{
+    lusgs_pre<LT, NEQ>(sm, d, w, rhs, btc, bjm, bjp, bkm, bkp, fac);
+    if (w.ev_fwd[0]) (void)hipEventRecord(w.ev_fwd[0], sm);
+    lusgs_band<LT, NEQ>(sm, d, w, true);
+    if (w.ev_fwd[1]) (void)hipEventRecord(w.ev_fwd[1], sm);
+    lusgs_mid<LT, NEQ>(sm, d, w);
+    lusgs_band<LT, NEQ>(sm, d, w, false);
+    lusgs_post<LT, NEQ>(sm, d, w, wk, q, dq, rhs, rhs_out);
 }
 
 template <typename LT>
@@ -809,6 +876,22 @@ void launch_lusgs1(hipStream_t sm, const Dims &d, const LusgsWs &w, const WakePa
 {
     launch_lusgs<double, 1>(sm, d, w, wk, q, dq, rhs, btc, bjm, bjp, bkm, bkp, fac, rhs_out);
 }
+
+#define PHASES(LT, NEQ)                                                                        \
+    template void lusgs_pre<LT, NEQ>(hipStream_t, const Dims &, const LusgsWs &, const double *, \
+                                     const LT *, const LT *, const LT *, const LT *, const LT *, \
+                                     LT *);                                                    \
+    template void lusgs_band<LT, NEQ>(hipStream_t, const Dims &, const LusgsWs &, bool);       \
+    template void lusgs_band_multi<LT, NEQ>(hipStream_t, const BandJob *, const int2 *, int,   \
+                                            int, bool);                                        \
+    template void lusgs_mid<LT, NEQ>(hipStream_t, const Dims &, const LusgsWs &);              \
+    template void lusgs_post<LT, NEQ>(hipStream_t, const Dims &, const LusgsWs &,              \
+                                      const WakePairs &, const double *, double *, double *,   \
+                                      bool);
+PHASES(double, 3)
+PHASES(float, 3)
+PHASES(double, 1)
+#undef PHASES
 
 template void launch_lusgs3<double>(hipStream_t, const Dims &, const LusgsWs &, const WakePairs &,
                                     const double *, double *, double *, const double *,

@@ -68,6 +68,28 @@ struct LusgsWs {
     int pf3, pf1;                // prefetch distance (steps) of the 3x3 and scalar sweeps
     hipEvent_t ev_fwd[2];        // when set: recorded around the forward band sweep
 };
+// one case's part of a multi-case band launch
+struct BandJob {
+    Dims d;
+    LusgsWs w;
+};
+// the solve in phases, for callers that put the band sweeps of several cases in one launch:
+// pre (transpose and factor), band (forward / backward), mid, post (back to the natural
+// layout, wake rows); launch_lusgs3 / launch_lusgs1 run all of them for one case
+template <typename LT, int NEQ>
+void lusgs_pre(hipStream_t, const Dims &, const LusgsWs &, const double *rhs, const LT *btc,
+               const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *fac);
+template <typename LT, int NEQ>
+void lusgs_band(hipStream_t, const Dims &, const LusgsWs &, bool fwd);
+// jobs and map in device memory; map[i] = (job, band in dispatch order), nblk entries
+template <typename LT, int NEQ>
+void lusgs_band_multi(hipStream_t, const BandJob *jobs, const int2 *map, int nblk, int pf,
+                      bool fwd);
+template <typename LT, int NEQ>
+void lusgs_mid(hipStream_t, const Dims &, const LusgsWs &);
+template <typename LT, int NEQ>
+void lusgs_post(hipStream_t, const Dims &, const LusgsWs &, const WakePairs &, const double *q,
+                double *dq, double *rhs, bool rhs_out);
 void lusgs_ws_shape(int jm, int km, int rows, LusgsWs &);
 long lusgs_ws_lhs_bytes(const LusgsWs &, int neqs);   // the five block groups
 long lusgs_ws_vec_bytes(const LusgsWs &, int neqs);   // one vector (rhs or dq)

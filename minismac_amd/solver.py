@@ -165,8 +165,8 @@ class Solver:
 
 
 class Batch:
-    """Several Solvers advanced together: after each case's first pass, one pass of all of
-    them is one launch of a captured HIP graph (smac_batch_run)."""
+    """Several Solvers advanced together on one stream, the LU-SGS band sweeps of all of them
+    in one launch per direction (smac_batch_run)."""
 
     def __init__(self, solvers):
         self.solvers = list(solvers)
