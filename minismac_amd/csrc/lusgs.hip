@@ -373,6 +373,22 @@ SMAC_DEV void edge_st(double v, rsrc_t r, unsigned vo, unsigned so)
                                           EDGE_POLICY);
 }
 
+// two adjacent slot words in one access (each word is still checked on its own)
+struct U2 {
+    unsigned long long x, y;
+};
+SMAC_DEV U2 edge_ld2(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(U2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so,
+                                                                        EDGE_POLICY));
+}
+
+SMAC_DEV void edge_st2(double a, double b, rsrc_t r, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, (D2{a, b})), r, (int)vo,
+                                           (int)so, EDGE_POLICY);
+}
+
 // Re-reads one hand-over slot and waits for it.  Written as one asm block so that the
 // compiler's wait-count bookkeeping never sees a load on the (rare) polling path: a load
 // there, with the full wait it needs, would otherwise degrade the waits the prefetch ring of
@@ -490,8 +506,14 @@ struct Band {
         ld_group<NEQ, NEQ>(sl.r, rR, v, pstride);
         // padded: rows outside 0..jm-1 read finite padding, bands without a neighbour read
         // a region of zeros
-#pragma unroll
-        for (int m = 0; m < NEQ; ++m) sl.e[m] = edge_ld(rEin, m * 8, so);
+        if constexpr (NEQ == 3) {
+            const U2 e01 = edge_ld2(rEin, 0, so);
+            sl.e[0] = e01.x;
+            sl.e[1] = e01.y;
+            sl.e[2] = edge_ld(rEin, 16, so);
+        } else {
+            sl.e[0] = edge_ld(rEin, 0, so);
+        }
     }
 
     // JSEL: some lane may sit on the first (FWD) / last column, where the reference does not
@@ -619,8 +641,12 @@ struct Band {
             o[0] = c.f[0] * t;
         }
         st_vec<NEQ>(o, rO, vo, pstride);   // inactive positions are padding
-#pragma unroll
-        for (int m = 0; m < NEQ; ++m) edge_st((double)o[m], rEout, vout + m * 8, eout);
+        if constexpr (NEQ == 3) {
+            edge_st2((double)o[0], (double)o[1], rEout, vout, eout);
+            edge_st((double)o[2], rEout, vout + 16, eout);
+        } else {
+            edge_st((double)o[0], rEout, vout, eout);
+        }
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) own[m] = o[m];
         load(c, vp, ein_pf);
