@@ -637,9 +637,15 @@ struct BandGroup {
     int nblk = 0, pf = 3;
 };
 
+// streams over which the per-case kernels of a batch phase are spread
+constexpr int BATCH_LANES = 8;
+
 struct smac_batch {
     std::vector<smac_handle *> cases;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;                 // band launches; phases fork from and join it
+    hipStream_t lane[BATCH_LANES] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[BATCH_LANES] = {};
     BandGroup sa;                    // SA LU-SGS (neqs 1, FP64) of the LU-SGS cases
     BandGroup mean[2];               // mean-flow LU-SGS: FP64, FP32-LHS
     std::vector<LusgsWs> snap_mean, snap_sa;   // workspaces the tables were built from
@@ -670,6 +676,11 @@ static void free_batch(smac_batch *b)
     free_group(b->sa);
     free_group(b->mean[0]);
     free_group(b->mean[1]);
+    for (int k = 0; k < BATCH_LANES; ++k) {
+        if (b->lane[k]) (void)hipStreamDestroy(b->lane[k]);
+        if (b->ev_join[k]) (void)hipEventDestroy(b->ev_join[k]);
+    }
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -746,91 +757,137 @@ static smac_status check_tables(smac_batch *b)
     return SMAC_OK;
 }
 
-// one pass of every case, in the order of enqueue_pass for each case
+// Phase boundaries: the lanes start after the batch stream's work so far (fork), and the
+// batch stream continues after all of theirs (join).
+static smac_status fork_lanes(smac_batch *b)
+{
+    HIP_TRY(hipEventRecord(b->ev_fork, b->stream));
+    for (int k = 0; k < BATCH_LANES; ++k) HIP_TRY(hipStreamWaitEvent(b->lane[k], b->ev_fork, 0));
+    return SMAC_OK;
+}
+
+static smac_status join_lanes(smac_batch *b)
+{
+    for (int k = 0; k < BATCH_LANES; ++k) {
+        HIP_TRY(hipEventRecord(b->ev_join[k], b->lane[k]));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
+    }
+    return SMAC_OK;
+}
+
+// one pass of every case, in the order of enqueue_pass for each case.  Case i's kernels of a
+// phase run on lane i % BATCH_LANES; the band sweeps of all cases on the batch stream.
 static smac_status enqueue_batch_pass(smac_batch *b)
 {
     hipStream_t sm = b->stream;
+    auto lane_of = [&](int i) { return b->lane[i % BATCH_LANES]; };
+    const int nc = (int)b->cases.size();
     // SA up to its band sweeps (line-relaxation cases: the whole SA step)
-    for (smac_handle *h : b->cases) {
+    B_TRY(fork_lanes(b));
+    for (int i = 0; i < nc; ++i) {
+        smac_handle *h = b->cases[i];
+        hipStream_t ls = lane_of(i);
         h->nt += 1;
         if (!has_sa(h)) continue;
         const Dims &d = h->d;
         SaArrays &a = h->sa;
-        if (h->first_sa) launch_sa_first(sm, d, h->walls, h->x, h->y, a);
-        launch_sa_assemble(sm, d, h->dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
+        if (h->first_sa) launch_sa_first(ls, d, h->walls, h->x, h->y, a);
+        launch_sa_assemble(ls, d, h->dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
                            (int)h->regs.size(), h->regs.data(), a);
         h->first_sa = false;
         if (is_lusgs(h)) {
-            lusgs_pre<double, 1>(sm, d, h->lws_sa, a.sf, a.stc, a.sjm, a.sjp, a.skm, a.skp, nullptr);
+            lusgs_pre<double, 1>(ls, d, h->lws_sa, a.sf, a.stc, a.sjm, a.sjp, a.skm, a.skp, nullptr);
         } else {
-            launch_sa_solve(sm, d, h->dp, h->p.ntjsp, h->wake, &h->lws_sa, a);
-            launch_sa_update(sm, d, h->dp, h->w, h->st, a);
+            launch_sa_solve(ls, d, h->dp, h->p.ntjsp, h->wake, &h->lws_sa, a);
+            launch_sa_update(ls, d, h->dp, h->w, h->st, a);
         }
     }
+    B_TRY(join_lanes(b));
     BandGroup &gs = b->sa;
-    lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true);
-    for (int i : gs.members) lusgs_mid<double, 1>(sm, b->cases[i]->d, b->cases[i]->lws_sa);
-    lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false);
-    for (int i : gs.members) {
-        smac_handle *h = b->cases[i];
-        SaArrays &a = h->sa;
-        lusgs_post<double, 1>(sm, h->d, h->lws_sa, h->wake, a.anut, a.ds, a.sf, false);
-        launch_sa_update(sm, h->d, h->dp, h->w, h->st, a);
+    if (gs.nblk) {
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true);
+        B_TRY(fork_lanes(b));
+        for (size_t n = 0; n < gs.members.size(); ++n) {
+            smac_handle *h = b->cases[gs.members[n]];
+            lusgs_mid<double, 1>(lane_of(gs.members[n]), h->d, h->lws_sa);
+        }
+        B_TRY(join_lanes(b));
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false);
     }
-    // mean flow up to its band sweeps (line-relaxation cases: the whole solve)
-    for (smac_handle *h : b->cases) {
+    // SA after its sweeps; mean flow up to its band sweeps (line relaxation: the whole solve)
+    B_TRY(fork_lanes(b));
+    for (int i = 0; i < nc; ++i) {
+        smac_handle *h = b->cases[i];
+        hipStream_t ls = lane_of(i);
         const Dims &d = h->d;
         const DevParams &dp = h->dp;
+        if (has_sa(h) && is_lusgs(h)) {
+            SaArrays &a = h->sa;
+            lusgs_post<double, 1>(ls, d, h->lws_sa, h->wake, a.anut, a.ds, a.sf, false);
+            launch_sa_update(ls, d, dp, h->w, h->st, a);
+        }
         double *btc = h->blk[0], *bjm = h->blk[1], *bjp = h->blk[2], *bkm = h->blk[3],
                *bkp = h->blk[4];
         const bool f32 = h->p.precision == SMAC_FP32_LHS;
         float *fb[5] = {(float *)btc, (float *)bjm, (float *)bjp, (float *)bkm, (float *)bkp};
         if (f32) {
-            launch_assemble<float>(sm, d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2],
+            launch_assemble<float>(ls, d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2],
                                    fb[3], fb[4]);
             for (const BcRegion &r : h->regs)
-                launch_bc<float>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, fb[0],
+                launch_bc<float>(ls, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, fb[0],
                                  fb[1], fb[2], fb[3], fb[4]);
         } else {
-            launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm,
+            launch_assemble<double>(ls, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm,
                                     bkp);
             for (const BcRegion &r : h->regs)
-                launch_bc<double>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm,
+                launch_bc<double>(ls, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm,
                                   bjp, bkm, bkp);
         }
-        launch_resid(sm, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
+        launch_resid(ls, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
         if (!is_lusgs(h))
-            launch_lrelax_ref<double>(sm, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake,
+            launch_lrelax_ref<double>(ls, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake,
                                       h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lr_lines);
         else if (f32)
-            lusgs_pre<float, 3>(sm, d, h->lws, h->s, fb[0], fb[1], fb[2], fb[3], fb[4], nullptr);
+            lusgs_pre<float, 3>(ls, d, h->lws, h->s, fb[0], fb[1], fb[2], fb[3], fb[4], nullptr);
         else
-            lusgs_pre<double, 3>(sm, d, h->lws, h->s, btc, bjm, bjp, bkm, bkp, nullptr);
+            lusgs_pre<double, 3>(ls, d, h->lws, h->s, btc, bjm, bjp, bkm, bkp, nullptr);
     }
+    B_TRY(join_lanes(b));
     for (int g = 0; g < 2; ++g) {
         BandGroup &gm = b->mean[g];
         if (!gm.nblk) continue;
-        if (g == 0) {
+        if (g == 0)
             lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
-            for (int i : gm.members) lusgs_mid<double, 3>(sm, b->cases[i]->d, b->cases[i]->lws);
-            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
-        } else {
+        else
             lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
-            for (int i : gm.members) lusgs_mid<float, 3>(sm, b->cases[i]->d, b->cases[i]->lws);
-            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+        B_TRY(fork_lanes(b));
+        for (int i : gm.members) {
+            if (g == 0)
+                lusgs_mid<double, 3>(lane_of(i), b->cases[i]->d, b->cases[i]->lws);
+            else
+                lusgs_mid<float, 3>(lane_of(i), b->cases[i]->d, b->cases[i]->lws);
         }
+        B_TRY(join_lanes(b));
+        if (g == 0)
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+        else
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
     }
-    for (smac_handle *h : b->cases) {
+    B_TRY(fork_lanes(b));
+    for (int i = 0; i < nc; ++i) {
+        smac_handle *h = b->cases[i];
+        hipStream_t ls = lane_of(i);
         const Dims &d = h->d;
         if (is_lusgs(h)) {
             if (h->p.precision == SMAC_FP32_LHS)
-                lusgs_post<float, 3>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, false);
+                lusgs_post<float, 3>(ls, d, h->lws, h->wake, h->q, h->dq, h->s, false);
             else
-                lusgs_post<double, 3>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, false);
+                lusgs_post<double, 3>(ls, d, h->lws, h->wake, h->q, h->dq, h->s, false);
         }
-        launch_update_force(sm, d, h->dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq,
+        launch_update_force(ls, d, h->dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq,
                             h->w, h->st);
     }
+    B_TRY(join_lanes(b));
     HIP_TRY(hipGetLastError());
     return SMAC_OK;
 }
@@ -857,9 +914,14 @@ smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out)
             }
         b->cases.push_back(h);
     }
-    if (hipStreamCreate(&b->stream) != hipSuccess) {
+    bool ok = hipStreamCreate(&b->stream) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < BATCH_LANES && ok; ++k)
+        ok = hipStreamCreate(&b->lane[k]) == hipSuccess &&
+             hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         free_batch(b);
-        return fail(SMAC_ERR_HIP, "hipStreamCreate failed");
+        return fail(SMAC_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
     smac_status rc = build_tables(b);
     if (rc) {
