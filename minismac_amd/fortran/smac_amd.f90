@@ -97,6 +97,37 @@ module smac_amd
       real(c_double), intent(out) :: q(*), anut(*), vnut(*)
     end function
 
+    integer(c_int) function smac_get_stats(h, st) bind(C, name="smac_get_stats")
+      import :: c_int, c_ptr, smac_iter_stats
+      type(c_ptr), value :: h
+      type(smac_iter_stats), intent(out) :: st
+    end function
+
+    integer(c_int) function smac_set_option(h, name, value) bind(C, name="smac_set_option")
+      import :: c_int, c_char, c_ptr
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: name(*)   ! NUL-terminated
+      integer(c_int), value :: value
+    end function
+
+    integer(c_int) function smac_batch_create(cases, n, b) bind(C, name="smac_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: cases(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: b
+    end function
+
+    integer(c_int) function smac_batch_run(b, npass) bind(C, name="smac_batch_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: npass
+    end function
+
+    integer(c_int) function smac_batch_destroy(b) bind(C, name="smac_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+
     type(c_ptr) function smac_last_error_c() bind(C, name="smac_last_error")
       import :: c_ptr
     end function
