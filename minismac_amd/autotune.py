@@ -6,6 +6,8 @@ are bit-identical for every setting):
 * ``lusgs_rows``        grid rows per wavefront band of the LU-SGS hyperplane sweeps;
 * ``lusgs_prefetch``    steps of operand prefetch in the 3x3 (mean-flow) band sweeps;
 * ``lusgs_prefetch_sa`` the same for the scalar (Spalart-Allmaras) band sweeps;
+* ``lusgs_ring``        1: the 3x3 sweeps stream their operands through an LDS ring fed by
+                        loader wavefronts (then ``lusgs_prefetch`` is not used);
 * ``lr_lines``          lines per workgroup of the line-relaxation factorisations.
 
 ``tune(case)`` times candidate settings on a private copy of the case (its own device
@@ -28,6 +30,7 @@ CACHE_ENV = "MINISMAC_AMD_TUNE_CACHE"
 LUSGS_ROWS = (16, 24, 32, 48, 64)
 LUSGS_PREFETCH = (2, 3, 4, 6, 8)
 LUSGS_PREFETCH_SA = (2, 4, 6, 8)
+LUSGS_RING = (0, 1)
 LR_LINES = (8, 16, 32, 64)
 # a setting replaces the current one only when it is faster by more than this fraction
 # (wall-clock noise of a few passes)
@@ -99,7 +102,8 @@ def _space(case: Case) -> list[tuple[str, tuple]]:
     p = case.params
     sa = p.ivis == 1 and p.iturb == 1
     if p.impsch == 2:
-        dims = [("lusgs_rows", LUSGS_ROWS), ("lusgs_prefetch", LUSGS_PREFETCH)]
+        dims = [("lusgs_ring", LUSGS_RING), ("lusgs_rows", LUSGS_ROWS),
+                ("lusgs_prefetch", LUSGS_PREFETCH)]
         if sa:
             dims.append(("lusgs_prefetch_sa", LUSGS_PREFETCH_SA))
         return dims
@@ -117,7 +121,8 @@ def tune(case: Case, passes: int = 10, repeats: int = 2, sweeps: int = 2,
         if hit is not None:
             return dict(hit["options"])
     dims = _space(case)
-    defaults = {"lusgs_rows": 64, "lusgs_prefetch": 4, "lusgs_prefetch_sa": 8, "lr_lines": 64}
+    defaults = {"lusgs_ring": 0, "lusgs_rows": 64, "lusgs_prefetch": 4, "lusgs_prefetch_sa": 8,
+                "lr_lines": 64}
     cur = {name: defaults[name] for name, _ in dims}
     probe = _Probe(case)
     try:

@@ -146,6 +146,7 @@ static smac_status alloc_lusgs_ws(smac_handle *h, int rows)
     h->lws.ev_fwd[1] = keep.ev_fwd[1];
     h->lws.pf3 = keep.pf3;
     h->lws.pf1 = keep.pf1;
+    h->lws.ring = keep.ring;
     h->lws_sa.pf3 = keep_sa.pf3;
     h->lws_sa.pf1 = keep_sa.pf1;
     smac_status rc = alloc_one_ws(h, rows, 3, h->lws);
@@ -381,6 +382,11 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
         if (!ok) return fail(SMAC_ERR_ARG, std::string(name) + ": unsupported prefetch distance");
         if (mean) h->lws.pf3 = value;
         else h->lws_sa.pf1 = value;
+        return SMAC_OK;
+    }
+    if (!std::strcmp(name, "lusgs_ring")) {
+        if (value != 0 && value != 1) return fail(SMAC_ERR_ARG, "lusgs_ring must be 0 or 1");
+        h->lws.ring = value;
         return SMAC_OK;
     }
     if (!std::strcmp(name, "lr_lines")) {
@@ -688,7 +694,8 @@ static void free_batch(smac_batch *b)
 static bool same_ws(const LusgsWs &a, const LusgsWs &c)
 {
     return a.lhs == c.lhs && a.r == c.r && a.dq == c.dq && a.edge == c.edge &&
-           a.nband == c.nband && a.rows == c.rows && a.pf3 == c.pf3 && a.pf1 == c.pf1;
+           a.nband == c.nband && a.rows == c.rows && a.pf3 == c.pf3 && a.pf1 == c.pf1 &&
+           a.ring == c.ring;
 }
 
 static smac_status build_group(BandGroup &g, const std::vector<BandJob> &jobs, int pf)
@@ -729,7 +736,7 @@ static smac_status build_tables(smac_batch *b)
         const int g = h->p.precision == SMAC_FP32_LHS ? 1 : 0;
         b->mean[g].members.push_back((int)i);
         jm[g].push_back(BandJob{h->d, w});
-        if (!pf_m[g]) pf_m[g] = w.pf3;
+        if (!pf_m[g]) pf_m[g] = w.ring ? -1 : w.pf3;
         if (has_sa(h)) {
             LusgsWs ws = h->lws_sa;
             ws.ev_fwd[0] = ws.ev_fwd[1] = nullptr;
@@ -741,8 +748,9 @@ static smac_status build_tables(smac_batch *b)
     // the prefetch distance of a multi-case launch is that of its first case (any value
     // gives the same results)
     B_TRY(build_group(b->sa, jsa, pf_sa ? pf_sa : 8));
-    B_TRY(build_group(b->mean[0], jm[0], pf_m[0] ? pf_m[0] : 4));
-    B_TRY(build_group(b->mean[1], jm[1], pf_m[1] ? pf_m[1] : 4));
+    // pf 0 selects the LDS-ring sweeps
+    B_TRY(build_group(b->mean[0], jm[0], pf_m[0] < 0 ? 0 : (pf_m[0] ? pf_m[0] : 4)));
+    B_TRY(build_group(b->mean[1], jm[1], pf_m[1] < 0 ? 0 : (pf_m[1] ? pf_m[1] : 4)));
     return SMAC_OK;
 }
 

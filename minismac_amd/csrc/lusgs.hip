@@ -467,6 +467,41 @@ SMAC_DEV void lu_solve3_fma(const float f[9], float r1, float r2, float r3, floa
     o[0] = __builtin_fmaf(-f[3], o[1], __builtin_fmaf(-f[6], o[2], z1));
 }
 
+// ---- LDS ring for the 3x3 sweeps ----------------------------------------------------------
+// The band's operands can be streamed into an LDS ring by RING_LOADERS loader wavefronts of
+// the same workgroup, RING_D steps ahead of the computing wavefront, with LDS DMA
+// (buffer_load ... lds): the memory-level parallelism of three wavefronts instead of one, and
+// no operand registers in flight.  Each loader issues every RING_LOADERS-th step and keeps
+// RING_M of its steps in flight beyond the two the computing wave is about to read, so one
+// constant vmcnt wait per step suffices; one s_barrier per step paces the waves.  A slot holds
+// the step's operand groups (64 lanes x 16 bytes each) and the 32-byte hand-over slot.
+constexpr int RING_LOADERS = 2;
+constexpr int RING_GROUP = BAND_LANES * 16;
+template <typename T> constexpr int ring_groups() { return 3 * pairs_of<3, T>(9) + pairs_of<3, T>(3); }
+template <typename T> constexpr int ring_m() { return sizeof(T) == 8 ? 2 : 4; }
+template <typename T> constexpr int ring_d() { return RING_LOADERS * ring_m<T>() + 2; }
+template <typename T> constexpr int ring_slots() { return ring_d<T>() + 1; }
+template <typename T> constexpr int ring_slot_bytes() { return ring_groups<T>() * RING_GROUP + 32; }
+template <typename T> constexpr int ring_bytes() { return ring_slots<T>() * ring_slot_bytes<T>(); }
+// VMEM instructions per issued step: the groups and two for the hand-over slot
+template <typename T> constexpr int ring_vm_per_step() { return ring_groups<T>() + 2; }
+static_assert((ring_m<double>() + 1) * ring_vm_per_step<double>() <= 63, "vmcnt budget");
+static_assert((ring_m<float>() + 1) * ring_vm_per_step<float>() <= 63, "vmcnt budget");
+static_assert(ring_d<float>() + 2 <= HP0 && ring_d<double>() + 2 <= HP0, "ring runs past padding");
+static_assert(ring_bytes<double>() <= 160 * 1024 && ring_bytes<float>() <= 160 * 1024, "LDS");
+
+// gfx9 s_waitcnt immediate: wait until at most n vector-memory operations are outstanding
+constexpr int vmcnt_imm(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+
+template <int AUX>
+SMAC_DEV void ring_dma(rsrc_t r, unsigned char *lds, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)lds, 16,
+                                             (int)vo, (int)so, 0, AUX);
+}
+
+SMAC_DEV void ring_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
 // Per-step operands of one lane, loaded PF steps ahead.
 template <int NEQ, typename T>
 struct Slot {
@@ -479,10 +514,12 @@ struct Slot {
 // ref/ilusol.f:353-469 and 560-591, from the middle-step rhs into the dq planes.
 // j-neighbour block: bjm (FWD) / bjp; k-neighbour block: bkm (FWD) / bkp.
 // T = double: the reference's arithmetic.  T = float (3x3 only): the FP32-LHS mode.
-template <int NEQ, int PF, bool FWD, typename T>
+template <int NEQ, int PF, bool FWD, typename T, bool RING = false>
 struct Band {
     static constexpr int NB = NEQ * NEQ;
     Dims d;
+    const unsigned char *ring;   // RING: the LDS ring (this lane's 16 bytes at lane * 16)
+    unsigned rnext;              // RING: byte offset of the slot the next refill reads
     int k, kin, kout;
     bool lane_in, kterm, broken;
     unsigned vout;      // producer lane: 0; others: out of range, so their stores drop
@@ -497,6 +534,35 @@ struct Band {
     int h;
     unsigned vo, vp;       // byte offset of this lane's position: current / prefetch step
     unsigned ein_pf, eout; // soffset of the hand-over slots: prefetch step / current step
+
+    // RING: one step's operands from LDS (the hand-over slot is read as a broadcast)
+    SMAC_DEV void ring_read(Slot<NEQ, T> &sl, unsigned off)
+    {
+        static_assert(NEQ == 3, "the ring serves the 3x3 sweeps");
+        constexpr int GB = pairs_of<3, T>(9), GR = pairs_of<3, T>(3), PW = pair_width<3, T>();
+        const unsigned char *b = ring + off + (threadIdx.x & 63) * 16;
+        T *dst[3] = {sl.f, sl.jn, sl.kn};
+#pragma unroll
+        for (int g = 0; g < 3; ++g)
+#pragma unroll
+            for (int p = 0; p < GB; ++p) {
+                const T *v = (const T *)(b + (g * GB + p) * RING_GROUP);
+#pragma unroll
+                for (int e = 0; e < PW; ++e)
+                    if (p * PW + e < 9) dst[g][p * PW + e] = v[e];
+            }
+#pragma unroll
+        for (int p = 0; p < GR; ++p) {
+            const T *v = (const T *)(b + (3 * GB + p) * RING_GROUP);
+#pragma unroll
+            for (int e = 0; e < PW; ++e)
+                if (p * PW + e < 3) sl.r[p * PW + e] = v[e];
+        }
+        const unsigned long long *e = (const unsigned long long *)(ring + off + ring_groups<T>() * RING_GROUP);
+        sl.e[0] = e[0];
+        sl.e[1] = e[1];
+        sl.e[2] = e[2];
+    }
 
     SMAC_DEV void load(Slot<NEQ, T> &sl, unsigned v, unsigned so)
     {
@@ -649,7 +715,14 @@ struct Band {
         }
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) own[m] = o[m];
-        load(c, vp, ein_pf);
+        if constexpr (RING) {
+            // the loaders have the step two ahead in LDS once every wave is past this point
+            ring_barrier();
+            ring_read(c, rnext);
+            rnext = rnext + ring_slot_bytes<T>() == ring_bytes<T>() ? 0u : rnext + ring_slot_bytes<T>();
+        } else {
+            load(c, vp, ein_pf);
+        }
         constexpr int ED = FWD ? 32 : -32;
         h += FWD ? 1 : -1;
         vo += dstep;
@@ -670,11 +743,11 @@ struct Band {
 
 // One band of one case: blk is the band's place in dispatch order (forward bands bottom up,
 // backward bands top down, so that every band is dispatched after the band it waits for).
-template <int NEQ, int PF, bool FWD, typename T>
+template <int NEQ, int PF, bool FWD, typename T, bool RING = false>
 SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
 {
     constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ, T>();
-    const int lane = threadIdx.x, nband = w.nband;
+    const int lane = threadIdx.x & 63, nband = w.nband;
     const int b = FWD ? blk : nband - 1 - blk;
     const int k0 = b * w.rows;
     const int klast = (k0 + w.rows < d.km ? k0 + w.rows : d.km) - 1;
@@ -684,7 +757,7 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
     const int dir = FWD ? 1 : -1;
     const int h0 = FWD ? hbeg : hend;
 
-    Band<NEQ, PF, FWD, T> B;
+    Band<NEQ, PF, FWD, T, RING> B;
     B.d = d;
     B.k = real ? k0 + lane : w.ks - 1;          // idle lanes run on a padding row
     // hand-over rows: the consumer row takes its k-neighbour from the next band over
@@ -729,11 +802,60 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
     B.ein_pf = ein0 + PF * dir * 32;
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) B.own[m] = (T)0;
-#pragma unroll
-    for (int u = 0; u < PF; ++u) B.load(B.s[u], B.vo + u * B.dstep, ein0 + u * dir * 32);
-
     // the first klast-k0+1 steps are the only ones where a lane meets the j boundary
     const int nch = (nsteps + PF - 1) / PF;
+    if constexpr (RING) {
+        static_assert(PF == 2 && NEQ == 3, "the ring feeds a two-slot register ring");
+        __shared__ __attribute__((aligned(16))) unsigned char ring_lds[ring_bytes<T>()];
+        const int wave = threadIdx.x >> 6;
+        if (wave > 0) {
+            // loader wavefront: steps s = wave - 1 (mod RING_LOADERS), RING_D ahead
+            constexpr int D = ring_d<T>(), SB = ring_slot_bytes<T>(), R = ring_slots<T>();
+            constexpr int GB = pairs_of<3, T>(9), GR = pairs_of<3, T>(3);
+            const int wl = wave - 1;
+            const rsrc_t rs[3] = {B.rF, B.rJ, B.rK};
+            auto issue = [&](int st) {
+                if (st % RING_LOADERS != wl) return;
+                unsigned char *sb = ring_lds + (st % R) * SB;
+                const unsigned v = B.vo + (unsigned)(st * B.dstep);
+#pragma unroll
+                for (int g = 0; g < 3; ++g)
+#pragma unroll
+                    for (int p = 0; p < GB; ++p)
+                        ring_dma<0>(rs[g], sb + (g * GB + p) * RING_GROUP, v, p * B.pstride);
+#pragma unroll
+                for (int p = 0; p < GR; ++p)
+                    ring_dma<0>(B.rR, sb + (3 * GB + p) * RING_GROUP, v, p * B.pstride);
+                // the hand-over slot: lane 0 alone, so 32 bytes land at the slot's end
+                const unsigned es = ein0 + (unsigned)(st * dir * 32);
+                unsigned char *eb = sb + ring_groups<T>() * RING_GROUP;
+                if (lane == 0) {
+                    ring_dma<EDGE_POLICY>(B.rEin, eb, 0, es);
+                    ring_dma<EDGE_POLICY>(B.rEin, eb + 16, 16, es);
+                }
+            };
+            constexpr int WAIT = vmcnt_imm(ring_m<T>() * ring_vm_per_step<T>());
+            for (int st = 0; st < D; ++st) issue(st);
+            __builtin_amdgcn_s_waitcnt(WAIT);
+            ring_barrier();
+            const int nexec = nch * PF;
+            for (int i = 0; i < nexec; ++i) {
+                issue(i + D);
+                __builtin_amdgcn_s_waitcnt(WAIT);
+                ring_barrier();
+            }
+            __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+            return;
+        }
+        B.ring = ring_lds;
+        ring_barrier();
+        B.ring_read(B.s[0], 0);
+        B.ring_read(B.s[1], ring_slot_bytes<T>());
+        B.rnext = 2 * ring_slot_bytes<T>();
+    } else {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) B.load(B.s[u], B.vo + u * B.dstep, ein0 + u * dir * 32);
+    }
     int nramp = (klast - k0 + 1 + PF - 1) / PF;
     nramp = nramp < nch ? nramp : nch;
     const bool krow = FWD ? k0 == 0 : klast == d.km - 1;
@@ -764,6 +886,24 @@ k_band_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map)
     const Dims d = jobs[m.x].d;
     const LusgsWs w = jobs[m.x].w;
     band_run<NEQ, PF, FWD, T>(d, w, m.y);
+}
+
+// the 3x3 sweeps fed through the LDS ring: one computing and RING_LOADERS loader wavefronts
+template <bool FWD, typename T>
+__global__ void __launch_bounds__(BAND_LANES * (1 + RING_LOADERS))
+k_band_ring(Dims d, LusgsWs w)
+{
+    band_run<3, 2, FWD, T, true>(d, w, (int)blockIdx.x);
+}
+
+template <bool FWD, typename T>
+__global__ void __launch_bounds__(BAND_LANES * (1 + RING_LOADERS))
+k_band_ring_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const Dims d = jobs[m.x].d;
+    const LusgsWs w = jobs[m.x].w;
+    band_run<3, 2, FWD, T, true>(d, w, m.y);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -825,6 +965,14 @@ void lusgs_pre(hipStream_t sm, const Dims &d, const LusgsWs &w, const double *rh
 template <typename LT, int NEQ>
 void lusgs_band(hipStream_t sm, const Dims &d, const LusgsWs &w, bool fwd)
 {
+    if (NEQ == 3 && w.ring) {
+        const dim3 blk(BAND_LANES * (1 + RING_LOADERS));
+        if (fwd)
+            hipLaunchKernelGGL((k_band_ring<true, LT>), dim3(w.nband), blk, 0, sm, d, w);
+        else
+            hipLaunchKernelGGL((k_band_ring<false, LT>), dim3(w.nband), blk, 0, sm, d, w);
+        return;
+    }
     const int pf = NEQ == 3 ? w.pf3 : w.pf1;
 #define ONE_F(PFV) hipLaunchKernelGGL((k_band<NEQ, PFV, true, LT>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
 #define ONE_B(PFV) hipLaunchKernelGGL((k_band<NEQ, PFV, false, LT>), dim3(w.nband), dim3(BAND_LANES), 0, sm, d, w)
@@ -841,6 +989,14 @@ void lusgs_band_multi(hipStream_t sm, const BandJob *jobs, const int2 *map, int 
                       bool fwd)
 {
     if (nblk <= 0) return;
+    if (NEQ == 3 && pf == 0) {   // pf 0 selects the LDS-ring sweeps
+        const dim3 blk(BAND_LANES * (1 + RING_LOADERS));
+        if (fwd)
+            hipLaunchKernelGGL((k_band_ring_multi<true, LT>), dim3(nblk), blk, 0, sm, jobs, map);
+        else
+            hipLaunchKernelGGL((k_band_ring_multi<false, LT>), dim3(nblk), blk, 0, sm, jobs, map);
+        return;
+    }
 #define MUL_F(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, true, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map)
 #define MUL_B(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, false, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map)
     if (fwd)

@@ -66,6 +66,7 @@ struct LusgsWs {
     unsigned long long *edge;    // band hand-over slots (forward half, backward half)
     int *stall;                  // raised when a hand-over wait times out
     int pf3, pf1;                // prefetch distance (steps) of the 3x3 and scalar sweeps
+    int ring;                    // 1: the 3x3 sweeps stream their operands through LDS
     hipEvent_t ev_fwd[2];        // when set: recorded around the forward band sweep
 };
 // one case's part of a multi-case band launch

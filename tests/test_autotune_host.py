@@ -9,11 +9,11 @@ from minismac_amd.case import config_case
 
 def test_search_space_follows_the_scheme():
     lu = dict(autotune._space(config_case(0, impsch=2)))
-    assert set(lu) == {"lusgs_rows", "lusgs_prefetch", "lusgs_prefetch_sa"}
+    assert set(lu) == {"lusgs_ring", "lusgs_rows", "lusgs_prefetch", "lusgs_prefetch_sa"}
     assert all(1 <= r <= 64 for r in lu["lusgs_rows"])
     assert set(lu["lusgs_prefetch"]) <= {1, 2, 3, 4, 6, 8}
     laminar = dict(autotune._space(config_case(0, impsch=2, iturb=0)))
-    assert set(laminar) == {"lusgs_rows", "lusgs_prefetch"}
+    assert set(laminar) == {"lusgs_ring", "lusgs_rows", "lusgs_prefetch"}
     assert set(dict(autotune._space(config_case(0, impsch=1)))) == {"lr_lines"}
 
 

@@ -1,0 +1,51 @@
+"""The LDS-ring form of the 3x3 LU-SGS band sweeps (option lusgs_ring): operands streamed into
+LDS by loader wavefronts.  It performs the same per-point arithmetic in the same order as the
+register-prefetch form, which the oracle pins, so the results must be bit-identical to it,
+for FP64 and FP32-LHS, one band and several, alone and in a batch."""
+import numpy as np
+import pytest
+
+from minismac_amd._abi import SMAC_FP32_LHS
+from minismac_amd.case import config_case, naca_case
+from minismac_amd.solver import Batch, Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(case, opts, n):
+    with Solver(case, options=opts) as s:
+        s.run(n)
+        q, a, v = s.get_state()
+        return q, a, v, s.stats()
+
+
+@pytest.mark.parametrize("precision", [0, SMAC_FP32_LHS])
+@pytest.mark.parametrize("shape,rows", [((945, 241), 64), ((239, 61), 16), ((239, 61), 64)])
+def test_ring_sweeps_match_register_sweeps(precision, shape, rows):
+    jm, km = shape
+    case = (config_case(1, impsch=2, precision=precision) if jm == 945 else
+            naca_case(jm, km, alpha=5.0, reynum=1e6, beta=10.0, impsch=2, precision=precision))
+    print(f"ring vs registers: {jm}x{km} rows={rows} precision={precision}", flush=True)
+    q0, a0, v0, s0 = _run(case, {"lusgs_rows": rows}, 6)
+    q1, a1, v1, s1 = _run(case, {"lusgs_rows": rows, "lusgs_ring": 1}, 6)
+    assert np.array_equal(q0, q1) and np.array_equal(a0, a1) and np.array_equal(v0, v1)
+    assert s0["rhsnorm"] == s1["rhsnorm"] and s0["cl"] == s1["cl"]
+
+
+def test_ring_in_a_batch_matches_register_sweeps_alone():
+    cases = [naca_case(239, 61, m=0.03, alpha=3.0, reynum=5e5, beta=10.0, impsch=2),
+             naca_case(477, 121, m=0.02, alpha=8.0, reynum=2e6, beta=10.0, impsch=2,
+                       precision=SMAC_FP32_LHS),
+             naca_case(239, 61, m=0.05, alpha=-2.0, reynum=3e6, beta=10.0, impsch=2,
+                       precision=SMAC_FP32_LHS)]
+    ref = [_run(c, {}, 5) for c in cases]
+    sol = [Solver(c, options={"lusgs_ring": 1}) for c in cases]
+    try:
+        with Batch(sol) as b:
+            b.run(5)
+        for (q0, a0, v0, s0), s in zip(ref, sol):
+            q1, a1, v1 = s.get_state()
+            assert np.array_equal(q0, q1) and np.array_equal(a0, a1) and np.array_equal(v0, v1)
+    finally:
+        for s in sol:
+            s.close()
