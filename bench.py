@@ -27,8 +27,8 @@ BAND_FWD_BYTES = {"fp64": 33 * 8, "fp32": 33 * 4}
 # HBM bytes per grid point that the same kernel moves, measured at 945x241 with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (scaled by 2.0 and 1.0: the factors that make
 # the two counters read 256 MiB for a plain 256 MiB device copy in the same run)
-BAND_FWD_TRAFFIC = {"fp64": (2.0 * 34421.5 + 8316.8) * 1024 / 227745,
-                    "fp32": (2.0 * 20273.5 + 4299.8) * 1024 / 227745}
+BAND_FWD_TRAFFIC = {"fp64": (69080.0 + 8225.0) * 1024 / 227745,
+                    "fp32": (40960.0 + 4207.0) * 1024 / 227745}
 
 
 def cpu_baseline(case, iters: int) -> dict:
@@ -177,7 +177,7 @@ def main() -> int:
         alg = BAND_FWD_BYTES["fp64"] * case.npts
         achieved = alg / (t * 1e-3) / 1e9
         line["roofline"] = {
-            "bound": "hbm", "kernel": "k_band<3,4,fwd,double> (LU-SGS forward band sweep)",
+            "bound": "hbm", "kernel": "k_band_ring<fwd,double> (LU-SGS forward band sweep, LDS-ring form)",
             "achieved": round(achieved, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 6),
             "traffic": round(BAND_FWD_TRAFFIC["fp64"] * case.npts),

@@ -147,8 +147,8 @@ smac_status smac_timings(smac_handle *h, double out[7]);
  * smac_get_array), "timing" (1: time kernel families with HIP events on the handle's
  * stream), "lusgs_rows" (1..64, default 64: grid rows per wavefront band of the LU-SGS
  * sweeps), "lusgs_prefetch" / "lusgs_prefetch_sa" (1, 2, 3, 4, 6 or 8; defaults 4 and 8:
- * steps of operand prefetch in the 3x3 / scalar LU-SGS sweeps), "lusgs_ring" (0 or 1: the
- * 3x3 sweeps stream their operands through an LDS ring fed by loader wavefronts),
+ * steps of operand prefetch in the 3x3 / scalar LU-SGS sweeps), "lusgs_ring" (0 or 1, default 1:
+ * the 3x3 sweeps stream their operands through an LDS ring fed by loader wavefronts),
  * "lr_lines" (1..64, default 64: lines per workgroup of the line-relaxation
  * factorisations).  All but the first two are tuning knobs; results do not depend on them.
  * No counterpart in the reference. */

@@ -121,7 +121,7 @@ def tune(case: Case, passes: int = 10, repeats: int = 2, sweeps: int = 2,
         if hit is not None:
             return dict(hit["options"])
     dims = _space(case)
-    defaults = {"lusgs_ring": 0, "lusgs_rows": 64, "lusgs_prefetch": 4, "lusgs_prefetch_sa": 8,
+    defaults = {"lusgs_ring": 1, "lusgs_rows": 64, "lusgs_prefetch": 4, "lusgs_prefetch_sa": 8,
                 "lr_lines": 64}
     cur = {name: defaults[name] for name, _ in dims}
     probe = _Probe(case)

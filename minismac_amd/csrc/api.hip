@@ -53,6 +53,7 @@ struct smac_handle {
     WakePairs wake = {};
     LusgsWs lws = {};          // LU-SGS sheared workspace (impsch = 2)
     int lusgs_rows = 64;
+    int lusgs_ring = 1;    // the 3x3 sweeps stream their operands through an LDS ring
     int lr_lines = 64;     // lines per workgroup of the line-relaxation factor kernels
     std::vector<void *> lws_allocs;
     Walls walls = {};
@@ -110,11 +111,12 @@ static void free_lusgs_ws(smac_handle *h)
 
 static smac_status alloc_one_ws(smac_handle *h, int rows, int neqs, LusgsWs &w)
 {
-    const int pf3 = w.pf3, pf1 = w.pf1;
+    const int pf3 = w.pf3, pf1 = w.pf1, ring = w.ring;
     hipEvent_t e0 = w.ev_fwd[0], e1 = w.ev_fwd[1];
     w = LusgsWs{};
     w.pf3 = pf3;
     w.pf1 = pf1;
+    w.ring = ring;
     w.ev_fwd[0] = e0;
     w.ev_fwd[1] = e1;
     lusgs_ws_shape(h->d.jm, h->d.km, rows, w);
@@ -146,7 +148,7 @@ static smac_status alloc_lusgs_ws(smac_handle *h, int rows)
     h->lws.ev_fwd[1] = keep.ev_fwd[1];
     h->lws.pf3 = keep.pf3;
     h->lws.pf1 = keep.pf1;
-    h->lws.ring = keep.ring;
+    h->lws.ring = h->lusgs_ring;
     h->lws_sa.pf3 = keep_sa.pf3;
     h->lws_sa.pf1 = keep_sa.pf1;
     smac_status rc = alloc_one_ws(h, rows, 3, h->lws);
@@ -386,6 +388,7 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
     }
     if (!std::strcmp(name, "lusgs_ring")) {
         if (value != 0 && value != 1) return fail(SMAC_ERR_ARG, "lusgs_ring must be 0 or 1");
+        h->lusgs_ring = value;
         h->lws.ring = value;
         return SMAC_OK;
     }
@@ -1095,6 +1098,7 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
     if (rc) return rc;
     LusgsWs ws{};
     lusgs_ws_shape(jmax, kmax, 64, ws);
+    ws.ring = 1;
     if (!lusgs_ws_fits(ws)) return fail(SMAC_ERR_SIZE, "grid too large for the LU-SGS workspace");
     const long lb = lusgs_ws_lhs_bytes(ws, neqs), vb = lusgs_ws_vec_bytes(ws, neqs);
     ws.lhs = b.get<char>(lb);

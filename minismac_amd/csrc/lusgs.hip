@@ -475,7 +475,7 @@ SMAC_DEV void lu_solve3_fma(const float f[9], float r1, float r2, float r3, floa
 // RING_M of its steps in flight beyond the two the computing wave is about to read, so one
 // constant vmcnt wait per step suffices; one s_barrier per step paces the waves.  A slot holds
 // the step's operand groups (64 lanes x 16 bytes each) and the 32-byte hand-over slot.
-constexpr int RING_LOADERS = 2;
+constexpr int RING_LOADERS = 3;
 constexpr int RING_GROUP = BAND_LANES * 16;
 template <typename T> constexpr int ring_groups() { return 3 * pairs_of<3, T>(9) + pairs_of<3, T>(3); }
 template <typename T> constexpr int ring_m() { return sizeof(T) == 8 ? 2 : 4; }
@@ -540,20 +540,23 @@ struct Band {
     {
         static_assert(NEQ == 3, "the ring serves the 3x3 sweeps");
         constexpr int GB = pairs_of<3, T>(9), GR = pairs_of<3, T>(3), PW = pair_width<3, T>();
-        const unsigned char *b = ring + off + (threadIdx.x & 63) * 16;
+        // 16-byte vector reads from one base with immediate offsets (ds_read_b128)
+        typedef T vec_t __attribute__((ext_vector_type(PW)));
+        const vec_t *b = (const vec_t *)(ring + off + (threadIdx.x & 63) * 16);
+        constexpr int GV = RING_GROUP / 16;   // vectors per group
         T *dst[3] = {sl.f, sl.jn, sl.kn};
 #pragma unroll
         for (int g = 0; g < 3; ++g)
 #pragma unroll
             for (int p = 0; p < GB; ++p) {
-                const T *v = (const T *)(b + (g * GB + p) * RING_GROUP);
+                const vec_t v = b[(g * GB + p) * GV];
 #pragma unroll
                 for (int e = 0; e < PW; ++e)
                     if (p * PW + e < 9) dst[g][p * PW + e] = v[e];
             }
 #pragma unroll
         for (int p = 0; p < GR; ++p) {
-            const T *v = (const T *)(b + (3 * GB + p) * RING_GROUP);
+            const vec_t v = b[(3 * GB + p) * GV];
 #pragma unroll
             for (int e = 0; e < PW; ++e)
                 if (p * PW + e < 3) sl.r[p * PW + e] = v[e];
