@@ -475,14 +475,12 @@ static smac_status enqueue_pass(smac_handle *h)
         float *fb[5] = {(float *)btc, (float *)bjm, (float *)bjp, (float *)bkm, (float *)bkp};
         launch_assemble<float>(sm, d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2], fb[3],
                                fb[4]);
-        for (const BcRegion &r : h->regs)
-            launch_bc<float>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, fb[0], fb[1],
-                             fb[2], fb[3], fb[4]);
+        launch_bc<float>(sm, d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y,
+                         h->met, h->q, h->s, fb[0], fb[1], fb[2], fb[3], fb[4]);
     } else {
         launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm, bkp);
-        for (const BcRegion &r : h->regs)
-            launch_bc<double>(sm, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm, bjp,
-                              bkm, bkp);
+        launch_bc<double>(sm, d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y,
+                          h->met, h->q, h->s, btc, bjm, bjp, bkm, bkp);
     }
     mark(h, 2);
     launch_resid(sm, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
@@ -844,15 +842,13 @@ static smac_status enqueue_batch_pass(smac_batch *b)
         if (f32) {
             launch_assemble<float>(ls, d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2],
                                    fb[3], fb[4]);
-            for (const BcRegion &r : h->regs)
-                launch_bc<float>(ls, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, fb[0],
-                                 fb[1], fb[2], fb[3], fb[4]);
+            launch_bc<float>(ls, d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x,
+                             h->y, h->met, h->q, h->s, fb[0], fb[1], fb[2], fb[3], fb[4]);
         } else {
             launch_assemble<double>(ls, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm,
                                     bkp);
-            for (const BcRegion &r : h->regs)
-                launch_bc<double>(ls, d, dp, r, h->st, h->x, h->y, h->met, h->q, h->s, btc, bjm,
-                                  bjp, bkm, bkp);
+            launch_bc<double>(ls, d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x,
+                              h->y, h->met, h->q, h->s, btc, bjm, bjp, bkm, bkp);
         }
         launch_resid(ls, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
         if (!is_lusgs(h))

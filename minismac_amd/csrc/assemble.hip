@@ -286,16 +286,22 @@ __global__ void __launch_bounds__(256) k_assemble(Dims d, DevParams prm,
 
 
 // ---------------------------------------------------------------------------------------
-// BC rows (ref/bc.f), one launch per region in file order
+// BC rows (ref/bc.f): every region of the case in one single-workgroup launch, region after
+// region in file order (a barrier between regions, so a later region's rows win where two
+// regions share a point, as in the reference's sequence of calls)
+
+constexpr int BC_THREADS = 256;
+constexpr int BC_MAX = 16;
+struct BcTable {
+    int n;
+    BcRegion r[BC_MAX];
+};
 
 // bcoutj, extrapolation branch of ibval 31 plus the fixed-pressure row (ref/bc.f:1627-1676)
 template <typename LT>
-__global__ void k_bc_outj(Dims d, int j, int kbeg, int kend, double pout,
-                          const double *__restrict__ q, double *__restrict__ s, LT *btc, LT *bjm,
-                          LT *bjp)
+SMAC_DEV void bc_outj(const Dims &d, int j, int k, double pout, const double *__restrict__ q,
+                      double *__restrict__ s, LT *btc, LT *bjm, LT *bjp)
 {
-    const int k = kbeg + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > kend) return;
     const long N = d.N, p = P(d, j, k);
     const int jn = j == 0 ? j + 1 : j - 1;
     LT *bn = j == 0 ? bjp : bjm;
@@ -312,11 +318,9 @@ __global__ void k_bc_outj(Dims d, int j, int kbeg, int kend, double pout,
 
 // bcnoslk, ibval 0 branch (ref/bc.f:1147-1158)
 template <typename LT>
-__global__ void k_bc_noslk(Dims d, int k, int jbeg, int jend, const double *__restrict__ q,
-                           double *__restrict__ s, LT *btc, LT *bkp)
+SMAC_DEV void bc_noslk(const Dims &d, int j, int k, const double *__restrict__ q,
+                       double *__restrict__ s, LT *btc, LT *bkp)
 {
-    const int j = jbeg + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j > jend) return;
     const long N = d.N, p = P(d, j, k), pu = P(d, j, k + 1);
     btc[p] = (LT)1.0;
     btc[4 * N + p] = (LT)1.0;
@@ -330,15 +334,12 @@ __global__ void k_bc_noslk(Dims d, int k, int jbeg, int jend, const double *__re
 // bccgridk, ibval 25 at k = kmax: free stream plus a point vortex of strength clift for
 // u, v and the outgoing characteristic for p (ref/bc.f:1334-1431)
 template <typename LT>
-__global__ void k_bc_cgridk(Dims d, int k, int jbeg, int jend, double alpha, double beta,
-                            const DevStats *__restrict__ st, const double *__restrict__ x,
-                            const double *__restrict__ y, const double *__restrict__ met,
-                            const double *__restrict__ q, double *__restrict__ s, LT *btc, LT *bkm)
+SMAC_DEV void bc_cgridk(const Dims &d, int j, int k, double alpha, double beta, double clift,
+                        const double *__restrict__ x, const double *__restrict__ y,
+                        const double *__restrict__ met, const double *__restrict__ q,
+                        double *__restrict__ s, LT *btc, LT *bkm)
 {
-    const int j = jbeg + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j > jend) return;
     const long N = d.N, p = P(d, j, k), pd = P(d, j, k - 1);
-    const double clift = st->clift;
     double pi = 4.0 * atan(1.0);
     double uinf = cos(alpha * pi / 180.);
     double vinf = sin(alpha * pi / 180.);
@@ -366,6 +367,30 @@ __global__ void k_bc_cgridk(Dims d, int k, int jbeg, int jend, double alpha, dou
     double dq2 = q[N + p] - q[N + pd];
     double dq3 = q[2 * N + p] - q[2 * N + pd];
     s[p] = b11 * dq1 + b12 * dq2 + b13 * dq3;
+}
+
+template <typename LT>
+__global__ void __launch_bounds__(BC_THREADS)
+k_bc(Dims d, BcTable t, double pout, double alpha, double beta, const DevStats *__restrict__ st,
+     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ met,
+     const double *__restrict__ q, double *__restrict__ s, LT *btc, LT *bjm, LT *bjp, LT *bkm,
+     LT *bkp)
+{
+    const double clift = st->clift;
+    for (int i = 0; i < t.n; ++i) {
+        const BcRegion r = t.r[i];
+        if (r.ibcval == 31) {
+            for (int k = r.kbeg + (int)threadIdx.x; k <= r.kend; k += BC_THREADS)
+                bc_outj<LT>(d, r.jbeg, k, pout, q, s, btc, bjm, bjp);
+        } else if (r.ibcval == 0) {
+            for (int j = r.jbeg + (int)threadIdx.x; j <= r.jend; j += BC_THREADS)
+                bc_noslk<LT>(d, j, r.kbeg, q, s, btc, bkp);
+        } else if (r.ibcval == 25) {
+            for (int j = r.jbeg + (int)threadIdx.x; j <= r.jend; j += BC_THREADS)
+                bc_cgridk<LT>(d, j, r.kbeg, alpha, beta, clift, x, y, met, q, s, btc, bkm);
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -487,24 +512,25 @@ void launch_assemble(hipStream_t sm, const Dims &d, const DevParams &prm, const 
 }
 
 template <typename LT>
-void launch_bc(hipStream_t sm, const Dims &d, const DevParams &prm, const BcRegion &r,
-               const DevStats *st, const double *x, const double *y, const double *met,
+void launch_bc(hipStream_t sm, const Dims &d, const DevParams &prm, const BcRegion *regs,
+               int nreg, const DevStats *st, const double *x, const double *y, const double *met,
                const double *q, double *s, LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp)
 {
-    if (r.ibcval == 31) {
-        int n = r.kend - r.kbeg + 1;
-        hipLaunchKernelGGL(k_bc_outj<LT>, dim3((n + 63) / 64), dim3(64), 0, sm, d, r.jbeg, r.kbeg,
-                           r.kend, prm.pout, q, s, btc, bjm, bjp);
-    } else if (r.ibcval == 0) {
-        int n = r.jend - r.jbeg + 1;
-        hipLaunchKernelGGL(k_bc_noslk<LT>, dim3((n + 63) / 64), dim3(64), 0, sm, d, r.kbeg, r.jbeg,
-                           r.jend, q, s, btc, bkp);
-    } else if (r.ibcval == 25) {
-        int n = r.jend - r.jbeg + 1;
-        hipLaunchKernelGGL(k_bc_cgridk<LT>, dim3((n + 63) / 64), dim3(64), 0, sm, d, r.kbeg,
-                           r.jbeg, r.jend, prm.alpha, prm.beta, st, x, y, met, q, s, btc, bkm);
+    // up to BC_MAX regions per launch; more go in further launches, still in file order
+    BcTable t{};
+    for (int i = 0; i <= nreg; ++i) {
+        if (i == nreg || t.n == BC_MAX) {
+            if (t.n)
+                hipLaunchKernelGGL(k_bc<LT>, dim3(1), dim3(BC_THREADS), 0, sm, d, t, prm.pout,
+                                   prm.alpha, prm.beta, st, x, y, met, q, s, btc, bjm, bjp, bkm,
+                                   bkp);
+            t.n = 0;
+            if (i == nreg) break;
+        }
+        const int ib = regs[i].ibcval;
+        if (ib != 31 && ib != 0 && ib != 25) continue;   // wake cuts: bcimpds in the solve
+        t.r[t.n++] = regs[i];
     }
-    (void)bkm;
 }
 
 void launch_resid(hipStream_t sm, const Dims &d, int nt, double beta, const double *s,
@@ -523,11 +549,11 @@ template void launch_assemble<double>(hipStream_t, const Dims &, const DevParams
 template void launch_assemble<float>(hipStream_t, const Dims &, const DevParams &,
                                      const double *, const double *, const double *, double *,
                                      float *, float *, float *, float *, float *);
-template void launch_bc<double>(hipStream_t, const Dims &, const DevParams &, const BcRegion &,
-                                const DevStats *, const double *, const double *, const double *,
-                                const double *, double *, double *, double *, double *, double *,
-                                double *);
-template void launch_bc<float>(hipStream_t, const Dims &, const DevParams &, const BcRegion &,
+template void launch_bc<double>(hipStream_t, const Dims &, const DevParams &, const BcRegion *,
+                                int, const DevStats *, const double *, const double *,
+                                const double *, const double *, double *, double *, double *,
+                                double *, double *, double *);
+template void launch_bc<float>(hipStream_t, const Dims &, const DevParams &, const BcRegion *, int,
                                const DevStats *, const double *, const double *, const double *,
                                const double *, double *, float *, float *, float *, float *,
                                float *);

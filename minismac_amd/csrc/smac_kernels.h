@@ -35,9 +35,9 @@ void launch_assemble(hipStream_t, const Dims &, const DevParams &, const double 
                      const double *met, const double *dj, double *s, LT *btc, LT *bjm, LT *bjp,
                      LT *bkm, LT *bkp);
 template <typename LT>
-void launch_bc(hipStream_t, const Dims &, const DevParams &, const BcRegion &, const DevStats *,
-               const double *x, const double *y, const double *met, const double *q, double *s,
-               LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
+void launch_bc(hipStream_t, const Dims &, const DevParams &, const BcRegion *regs, int nreg,
+               const DevStats *, const double *x, const double *y, const double *met,
+               const double *q, double *s, LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
 void launch_resid(hipStream_t, const Dims &, int nt, double beta, const double *s,
                   const double *dj, Scratch &, DevStats *);
 
