@@ -178,6 +178,7 @@ __global__ void __launch_bounds__(64) k_btund3(long N, int nline, int len, long 
 // workgroup; `dir` chooses xi lines (lines of constant k) or eta lines (constant j).
 
 constexpr int SWEEP_THREADS = 256;
+constexpr int SWEEP_CHUNK = 256;   // points of the band staged per chunk (two chunks in LDS)
 
 template <typename LT>
 __global__ void __launch_bounds__(SWEEP_THREADS)
@@ -187,7 +188,8 @@ k_sweep_ref(Dims d, int eta, int nsweep, double underr, double *__restrict__ dq,
 {
     // lm/lp: blocks along the line (bjm/bjp for xi, bkm/bkp for eta);
     // om/op: blocks to the neighbouring lines (bkm/bkp for xi, bjm/bjp for eta)
-    extern __shared__ double rhs[];   // 3 * len
+    extern __shared__ double lds[];   // 3 * len rhs, then two chunks of 18 band planes
+    double *rhs = lds, *band = lds + 3 * (eta ? d.km : d.jm);
     const long N = d.N;
     const int nline = eta ? d.jm : d.km;
     const int len = eta ? d.km : d.jm;
@@ -218,46 +220,105 @@ k_sweep_ref(Dims d, int eta, int nsweep, double underr, double *__restrict__ dq,
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            // forward with the factored diagonal blocks and the lower band (ref/lrelax.f:331-351)
-            double r1 = 0, r2 = 0, r3 = 0;
-            for (int i = 0; i < len; ++i) {
-                const long p = lstride * l + stride * i;
-                double t1 = rhs[3 * i], t2 = rhs[3 * i + 1], t3 = rhs[3 * i + 2];
-                if (i > 0) {
-                    LT a[9];
-                    ld9(lm, N, p, a);
-                    t1 = t1 - a[0] * r1 - a[3] * r2 - a[6] * r3;
-                    t2 = t2 - a[1] * r1 - a[4] * r2 - a[7] * r3;
-                    t3 = t3 - a[2] * r1 - a[5] * r2 - a[8] * r3;
+        // The solve along the line is one dependent chain, run by thread 0 from LDS: the
+        // band is staged in chunks of SWEEP_CHUNK points (the reference copies it to /btri/,
+        // ref/lrelax.f:310-321), the next chunk loaded by waves 1.. while thread 0 works
+        // through the current one.
+        // forward with the factored diagonal blocks and the lower band (ref/lrelax.f:331-351)
+        {
+            const int nch = (len + SWEEP_CHUNK - 1) / SWEEP_CHUNK;
+            auto stage = [&](int c, double *buf) {
+                for (int ii = (int)threadIdx.x - 64; ii < SWEEP_CHUNK; ii += SWEEP_THREADS - 64) {
+                    if (ii < 0) continue;
+                    const int i = c * SWEEP_CHUNK + ii;
+                    if (i >= len) break;
+                    const long p = lstride * l + stride * i;
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) {
+                        buf[e * SWEEP_CHUNK + ii] = (double)lm[e * N + p];
+                        buf[(9 + e) * SWEEP_CHUNK + ii] = (double)btc[e * N + p];
+                    }
                 }
-                LT b[9];
-                ld9(btc, N, p, b);
-                double d1 = b[0] * t1;
-                double d2 = b[4] * (t2 - b[1] * d1);
-                double d3 = b[8] * (t3 - b[2] * d1 - b[5] * d2);
-                r3 = d3;
-                r2 = d2 - b[7] * r3;
-                r1 = d1 - b[6] * r3 - b[3] * r2;
-                rhs[3 * i] = r1;
-                rhs[3 * i + 1] = r2;
-                rhs[3 * i + 2] = r3;
+            };
+            if (threadIdx.x >= 64) stage(0, band);
+            __syncthreads();
+            double r1 = 0, r2 = 0, r3 = 0;
+            for (int c = 0; c < nch; ++c) {
+                double *cur = band + (c & 1) * 18 * SWEEP_CHUNK;
+                if (threadIdx.x >= 64) {
+                    if (c + 1 < nch) stage(c + 1, band + ((c + 1) & 1) * 18 * SWEEP_CHUNK);
+                } else if (threadIdx.x == 0) {
+                    const int i1 = (c + 1) * SWEEP_CHUNK < len ? (c + 1) * SWEEP_CHUNK : len;
+#pragma unroll 4
+                    for (int i = c * SWEEP_CHUNK; i < i1; ++i) {
+                        const int ii = i - c * SWEEP_CHUNK;
+                        double t1 = rhs[3 * i], t2 = rhs[3 * i + 1], t3 = rhs[3 * i + 2];
+                        if (i > 0) {
+                            const double *a = cur + ii;
+                            t1 = t1 - a[0] * r1 - a[3 * SWEEP_CHUNK] * r2 - a[6 * SWEEP_CHUNK] * r3;
+                            t2 = t2 - a[1 * SWEEP_CHUNK] * r1 - a[4 * SWEEP_CHUNK] * r2 -
+                                 a[7 * SWEEP_CHUNK] * r3;
+                            t3 = t3 - a[2 * SWEEP_CHUNK] * r1 - a[5 * SWEEP_CHUNK] * r2 -
+                                 a[8 * SWEEP_CHUNK] * r3;
+                        }
+                        const double *bb = cur + 9 * SWEEP_CHUNK + ii;
+                        double b0 = bb[0], b1 = bb[SWEEP_CHUNK], b2 = bb[2 * SWEEP_CHUNK],
+                               b3 = bb[3 * SWEEP_CHUNK], b4 = bb[4 * SWEEP_CHUNK],
+                               b5 = bb[5 * SWEEP_CHUNK], b6 = bb[6 * SWEEP_CHUNK],
+                               b7 = bb[7 * SWEEP_CHUNK], b8 = bb[8 * SWEEP_CHUNK];
+                        double d1 = b0 * t1;
+                        double d2 = b4 * (t2 - b1 * d1);
+                        double d3 = b8 * (t3 - b2 * d1 - b5 * d2);
+                        r3 = d3;
+                        r2 = d2 - b7 * r3;
+                        r1 = d1 - b6 * r3 - b3 * r2;
+                        rhs[3 * i] = r1;
+                        rhs[3 * i + 1] = r2;
+                        rhs[3 * i + 2] = r3;
+                    }
+                }
+                __syncthreads();
             }
-            // back substitution with the upper band (ref/lrelax.f:367-373)
-            for (int i = len - 2; i >= 0; --i) {
-                const long p = lstride * l + stride * i;
-                LT c[9];
-                ld9(lp, N, p, c);
-                double t1 = rhs[3 * i] - c[0] * r1 - c[3] * r2 - c[6] * r3;
-                double t2 = rhs[3 * i + 1] - c[1] * r1 - c[4] * r2 - c[7] * r3;
-                double t3 = rhs[3 * i + 2] - c[2] * r1 - c[5] * r2 - c[8] * r3;
-                r1 = t1; r2 = t2; r3 = t3;
-                rhs[3 * i] = r1;
-                rhs[3 * i + 1] = r2;
-                rhs[3 * i + 2] = r3;
+            // back substitution with the upper band (ref/lrelax.f:367-373), chunks from the end
+            auto stage_c = [&](int c, double *buf) {
+                for (int ii = (int)threadIdx.x - 64; ii < SWEEP_CHUNK; ii += SWEEP_THREADS - 64) {
+                    if (ii < 0) continue;
+                    const int i = c * SWEEP_CHUNK + ii;
+                    if (i > len - 2) break;
+                    const long p = lstride * l + stride * i;
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) buf[e * SWEEP_CHUNK + ii] = (double)lp[e * N + p];
+                }
+            };
+            const int nb = len >= 2 ? (len - 2) / SWEEP_CHUNK + 1 : 0;   // chunks holding 0..len-2
+            if (nb > 0 && threadIdx.x >= 64) stage_c(nb - 1, band);
+            __syncthreads();
+            for (int q = 0; q < nb; ++q) {
+                const int c = nb - 1 - q;
+                double *cur = band + (q & 1) * 18 * SWEEP_CHUNK;
+                if (threadIdx.x >= 64) {
+                    if (c > 0) stage_c(c - 1, band + ((q + 1) & 1) * 18 * SWEEP_CHUNK);
+                } else if (threadIdx.x == 0) {
+                    const int ihi = (c + 1) * SWEEP_CHUNK - 1 < len - 2 ? (c + 1) * SWEEP_CHUNK - 1
+                                                                      : len - 2;
+#pragma unroll 4
+                    for (int i = ihi; i >= c * SWEEP_CHUNK; --i) {
+                        const double *cc = cur + (i - c * SWEEP_CHUNK);
+                        double t1 = rhs[3 * i] - cc[0] * r1 - cc[3 * SWEEP_CHUNK] * r2 -
+                                    cc[6 * SWEEP_CHUNK] * r3;
+                        double t2 = rhs[3 * i + 1] - cc[1 * SWEEP_CHUNK] * r1 -
+                                    cc[4 * SWEEP_CHUNK] * r2 - cc[7 * SWEEP_CHUNK] * r3;
+                        double t3 = rhs[3 * i + 2] - cc[2 * SWEEP_CHUNK] * r1 -
+                                    cc[5 * SWEEP_CHUNK] * r2 - cc[8 * SWEEP_CHUNK] * r3;
+                        r1 = t1; r2 = t2; r3 = t3;
+                        rhs[3 * i] = r1;
+                        rhs[3 * i + 1] = r2;
+                        rhs[3 * i + 2] = r3;
+                    }
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
         for (int i = threadIdx.x; i < len; i += SWEEP_THREADS) {
             const long p = lstride * l + stride * i;
 #pragma unroll
@@ -313,7 +374,7 @@ void launch_lrelax_ref(hipStream_t sm, const Dims &d, const DevParams &prm, int 
                            1L, (long)d.jm, bjm, btc, bjp);
     for (int nj = 1; nj <= njsp; ++nj) {
         hipLaunchKernelGGL(k_sweep_ref<LT>, dim3(1), dim3(SWEEP_THREADS),
-                           sizeof(double) * 3 * d.jm, sm, d, 0, nj, prm.underr, dq, s, btc, bjm,
+                           sizeof(double) * (3 * d.jm + 36 * SWEEP_CHUNK), sm, d, 0, nj, prm.underr, dq, s, btc, bjm,
                            bjp, bkm, bkp);
         if ((nj / nbcimp) * nbcimp == nj) launch_wake(sm, d, 3, w, q, dq, s);
     }
@@ -325,7 +386,7 @@ void launch_lrelax_ref(hipStream_t sm, const Dims &d, const DevParams &prm, int 
                            (long)d.jm, 1L, bkm, btc, bkp);
     for (int nk = 1; nk <= nksp; ++nk) {
         hipLaunchKernelGGL(k_sweep_ref<LT>, dim3(1), dim3(SWEEP_THREADS),
-                           sizeof(double) * 3 * d.km, sm, d, 1, nk, prm.underr, dq, s, btc, bkm,
+                           sizeof(double) * (3 * d.km + 36 * SWEEP_CHUNK), sm, d, 1, nk, prm.underr, dq, s, btc, bkm,
                            bkp, bjm, bjp);
         if ((nk / nbcimp) * nbcimp == nk) launch_wake(sm, d, 3, w, q, dq, s);
     }
