@@ -164,6 +164,26 @@ def test_gpu_wider_grid_and_other_settings():
     assert np.array_equal(q, o.array("q"))
 
 
+@pytest.mark.parametrize("jm", [259, 513, 519])
+def test_gpu_line_relaxation_long_lines(jm):
+    """xi lines longer than one LDS chunk of the sequential line solve (256 points): chunk
+    boundaries inside the forward and the backward substitution (513: the forward pass ends
+    with a one-point chunk and the backward pass with a full one; 259, 519: both end with
+    partial chunks), bit for bit."""
+    from minismac_amd import grid
+    case = config_case(0, impsch=1)
+    g0 = grid.make_cgrid(jm, 41, wall_spacing=1e-5)
+    case.x, case.y = g0.x, g0.y
+    case.regions = grid.cgrid_bc_regions(g0.jte_lower, g0.jte_upper)
+    print(f"line relaxation {jm}x41", flush=True)
+    o = pyoracle.Oracle(case)
+    with Solver(case) as g:
+        for it in range(4):
+            _compare(g.step(), o.step(), it + 1)
+        q, _, _ = g.get_state()
+    assert np.array_equal(q, o.array("q"))
+
+
 def test_gpu_runs_are_deterministic():
     case = config_case(0, impsch=2)
     outs = []
