@@ -122,7 +122,8 @@ smac_status smac_get_stats(smac_handle *h, smac_iter_stats *st);
  * kernels of each case are enqueued between them.  Each case gets exactly the kernels and
  * operands of its own pass: its results are those of smac_step on it alone.  The cases may
  * differ in size, scheme, settings and precision; options "timing" and "keep_lhs" must be
- * off.  A case must not be stepped on its own while a smac_batch_run on it is in progress. */
+ * off; all must live on the device that is current when the batch is created.  A case must
+ * not be stepped on its own while a smac_batch_run on it is in progress. */
 typedef struct smac_batch smac_batch;
 smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out);
 /* npass passes of every case; returns after the device has finished them, having checked

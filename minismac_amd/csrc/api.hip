@@ -746,8 +746,8 @@ static smac_status build_tables(smac_batch *b)
             if (!pf_sa) pf_sa = ws.pf1;
         }
     }
-    // the prefetch distance of a multi-case launch is that of its first case (any value
-    // gives the same results)
+    // the sweep form (LDS ring or register prefetch) and prefetch distance of a multi-case
+    // launch are those of its first case (every form and distance gives the same results)
     B_TRY(build_group(b->sa, jsa, pf_sa ? pf_sa : 8));
     // pf 0 selects the LDS-ring sweeps
     B_TRY(build_group(b->mean[0], jm[0], pf_m[0] < 0 ? 0 : (pf_m[0] ? pf_m[0] : 4)));
