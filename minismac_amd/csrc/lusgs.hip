@@ -89,6 +89,17 @@ SMAC_DEV void lu_solve3(const AT f[9], AT r1, AT r2, AT r3, AT o[3])
     o[0] = z1 - f[6] * o[2] - f[3] * o[1];
 }
 
+// FP32-LHS form of the 3x3 substitution (fused multiply-adds; not the reference's rounding)
+SMAC_DEV void lu_solve3_fma(const float f[9], float r1, float r2, float r3, float o[3])
+{
+    const float z1 = r1 * f[0];
+    const float z2 = __builtin_fmaf(-f[1], z1, r2) * f[4];
+    const float z3 = __builtin_fmaf(-f[5], z2, __builtin_fmaf(-f[2], z1, r3)) * f[8];
+    o[2] = z3;
+    o[1] = __builtin_fmaf(-f[7], o[2], z2);
+    o[0] = __builtin_fmaf(-f[3], o[1], __builtin_fmaf(-f[6], o[2], z1));
+}
+
 // ------------------------------------------------------------------------------------------
 // natural <-> sheared transposition through LDS tiles of TK rows x TJ columns
 
@@ -148,6 +159,15 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         }
     }
 
+    // The FP32-LHS sweeps use the scaled form of the same operator: with D = LU(btc),
+    // (D+L) y = r is y = D^-1 r - (D^-1 L) y, and (D+U) x = D y is x = y - (D^-1 U) x, so the
+    // four neighbour blocks and the rhs are stored premultiplied by D^-1 (three substitutions
+    // per block), and neither the sweeps nor the middle step need the factors.
+    constexpr bool SCALED = NEQ == 3 && sizeof(LT) == 4;
+    constexpr int NS = TK * TJ / TILE_THREADS;   // points of the tile per thread
+    static_assert(TK * TJ % TILE_THREADS == 0, "tile points per thread");
+    float fs[SCALED ? NS : 1][9];
+
     // group 0: diagonal factors
     for (int e = threadIdx.x; e < TK * TJ; e += TILE_THREADS) {
         const int kk = e / TJ, jj = e % TJ, j = j0 + jj, k = k0 + kk;
@@ -161,6 +181,10 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         } else {
             f[0] = (LT)1. / t[0];   // lusini1 (ref/lusgs.f:230-234)
         }
+        if constexpr (SCALED) {
+#pragma unroll
+            for (int m = 0; m < 9; ++m) fs[(e - (int)threadIdx.x) / TILE_THREADS][m] = (float)f[m];
+        }
 #pragma unroll
         for (int m = 0; m < NB; ++m) {
             lds[m][kk][jj] = (double)f[m];
@@ -168,7 +192,24 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         }
     }
     __syncthreads();
-    tile_to_sheared<LT, NEQ, NB>(w, d.jm, d.km, j0, k0, lds, wl);
+    if constexpr (!SCALED) tile_to_sheared<LT, NEQ, NB>(w, d.jm, d.km, j0, k0, lds, wl);
+
+    // scaled form: this thread's points of the tile in LDS, NP planes (blocks: 3 columns)
+    auto scale = [&](int np) {
+        for (int e = threadIdx.x; e < TK * TJ; e += TILE_THREADS) {
+            const int kk = e / TJ, jj = e % TJ, j = j0 + jj, k = k0 + kk;
+            if (j >= d.jm || k >= d.km) continue;
+            const float *f = fs[(e - (int)threadIdx.x) / TILE_THREADS];
+            for (int c = 0; c < np / 3; ++c) {
+                float o[3];
+                lu_solve3_fma(f, (float)lds[3 * c][kk][jj], (float)lds[3 * c + 1][kk][jj],
+                              (float)lds[3 * c + 2][kk][jj], o);
+                lds[3 * c][kk][jj] = (double)o[0];
+                lds[3 * c + 1][kk][jj] = (double)o[1];
+                lds[3 * c + 2][kk][jj] = (double)o[2];
+            }
+        }
+    };
 
     const LT *src[4] = {bjm, bkm, bjp, bkp};
 #pragma unroll
@@ -176,11 +217,19 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         __syncthreads();
         tile_from_natural<LT, NB>(d, j0, k0, lds, src[g]);
         __syncthreads();
+        if constexpr (SCALED) {
+            scale(9);
+            __syncthreads();
+        }
         tile_to_sheared<LT, NEQ, NB>(w, d.jm, d.km, j0, k0, lds, wl + (g + 1) * gstride);
     }
     __syncthreads();
     tile_from_natural<double, NEQ>(d, j0, k0, lds, rhs);
     __syncthreads();
+    if constexpr (SCALED) {
+        scale(3);
+        __syncthreads();
+    }
     tile_to_sheared<LT, NEQ, NEQ>(w, d.jm, d.km, j0, k0, lds, (LT *)w.r);   // the solve's type
 }
 
@@ -456,17 +505,6 @@ SMAC_DEV T keep(T x)
     return x;
 }
 
-// FP32-LHS form of the 3x3 substitution (fused multiply-adds; not the reference's rounding)
-SMAC_DEV void lu_solve3_fma(const float f[9], float r1, float r2, float r3, float o[3])
-{
-    const float z1 = r1 * f[0];
-    const float z2 = __builtin_fmaf(-f[1], z1, r2) * f[4];
-    const float z3 = __builtin_fmaf(-f[5], z2, __builtin_fmaf(-f[2], z1, r3)) * f[8];
-    o[2] = z3;
-    o[1] = __builtin_fmaf(-f[7], o[2], z2);
-    o[0] = __builtin_fmaf(-f[3], o[1], __builtin_fmaf(-f[6], o[2], z1));
-}
-
 // ---- LDS ring for the 3x3 sweeps ----------------------------------------------------------
 // The band's operands can be streamed into an LDS ring by RING_LOADERS loader wavefronts of
 // the same workgroup, RING_D steps ahead of the computing wavefront, with LDS DMA
@@ -477,7 +515,12 @@ SMAC_DEV void lu_solve3_fma(const float f[9], float r1, float r2, float r3, floa
 // the step's operand groups (64 lanes x 16 bytes each) and the 32-byte hand-over slot.
 constexpr int RING_LOADERS = 3;
 constexpr int RING_GROUP = BAND_LANES * 16;
-template <typename T> constexpr int ring_groups() { return 3 * pairs_of<3, T>(9) + pairs_of<3, T>(3); }
+// FP32-LHS sweeps run in the scaled form (the blocks and the rhs premultiplied by the inverse
+// diagonal): they read no diagonal factors
+template <typename T> constexpr bool scaled_form() { return sizeof(T) == 4; }
+// block groups a step reads: factors (FP64 only), j-neighbour and k-neighbour blocks
+template <typename T> constexpr int ring_blocks() { return scaled_form<T>() ? 2 : 3; }
+template <typename T> constexpr int ring_groups() { return ring_blocks<T>() * pairs_of<3, T>(9) + pairs_of<3, T>(3); }
 template <typename T> constexpr int ring_m() { return sizeof(T) == 8 ? 2 : 4; }
 template <typename T> constexpr int ring_d() { return RING_LOADERS * ring_m<T>() + 2; }
 template <typename T> constexpr int ring_slots() { return ring_d<T>() + 1; }
@@ -544,19 +587,20 @@ struct Band {
         typedef T vec_t __attribute__((ext_vector_type(PW)));
         const vec_t *b = (const vec_t *)(ring + off + (threadIdx.x & 63) * 16);
         constexpr int GV = RING_GROUP / 16;   // vectors per group
+        constexpr int NBK = ring_blocks<T>();
         T *dst[3] = {sl.f, sl.jn, sl.kn};
 #pragma unroll
-        for (int g = 0; g < 3; ++g)
+        for (int g = 0; g < NBK; ++g)
 #pragma unroll
             for (int p = 0; p < GB; ++p) {
                 const vec_t v = b[(g * GB + p) * GV];
 #pragma unroll
                 for (int e = 0; e < PW; ++e)
-                    if (p * PW + e < 9) dst[g][p * PW + e] = v[e];
+                    if (p * PW + e < 9) dst[g + 3 - NBK][p * PW + e] = v[e];
             }
 #pragma unroll
         for (int p = 0; p < GR; ++p) {
-            const vec_t v = b[(3 * GB + p) * GV];
+            const vec_t v = b[(NBK * GB + p) * GV];
 #pragma unroll
             for (int e = 0; e < PW; ++e)
                 if (p * PW + e < 3) sl.r[p * PW + e] = v[e];
@@ -569,7 +613,7 @@ struct Band {
 
     SMAC_DEV void load(Slot<NEQ, T> &sl, unsigned v, unsigned so)
     {
-        ld_group<NEQ, NB>(sl.f, rF, v, pstride);
+        if constexpr (!(NEQ == 3 && scaled_form<T>())) ld_group<NEQ, NB>(sl.f, rF, v, pstride);
         ld_group<NEQ, NB>(sl.jn, rJ, v, pstride);
         ld_group<NEQ, NB>(sl.kn, rK, v, pstride);
         ld_group<NEQ, NEQ>(sl.r, rR, v, pstride);
@@ -644,7 +688,10 @@ struct Band {
                 kpart();
                 jpart();
             }
-            lu_solve3_fma(c.f, r[0], r[1], r[2], o);
+            // scaled form: the blocks and the rhs already carry the inverse diagonal
+            o[0] = r[0];
+            o[1] = r[1];
+            o[2] = r[2];
         } else if constexpr (NEQ == 3) {
             double r[3] = {c.r[0], c.r[1], c.r[2]};
             auto jpart = [&]() {
@@ -816,19 +863,21 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
             constexpr int D = ring_d<T>(), SB = ring_slot_bytes<T>(), R = ring_slots<T>();
             constexpr int GB = pairs_of<3, T>(9), GR = pairs_of<3, T>(3);
             const int wl = wave - 1;
+            constexpr int NBK = ring_blocks<T>();
             const rsrc_t rs[3] = {B.rF, B.rJ, B.rK};
             auto issue = [&](int st) {
                 if (st % RING_LOADERS != wl) return;
                 unsigned char *sb = ring_lds + (st % R) * SB;
                 const unsigned v = B.vo + (unsigned)(st * B.dstep);
 #pragma unroll
-                for (int g = 0; g < 3; ++g)
+                for (int g = 0; g < NBK; ++g)
 #pragma unroll
                     for (int p = 0; p < GB; ++p)
-                        ring_dma<0>(rs[g], sb + (g * GB + p) * RING_GROUP, v, p * B.pstride);
+                        ring_dma<0>(rs[g + 3 - NBK], sb + (g * GB + p) * RING_GROUP, v,
+                                    p * B.pstride);
 #pragma unroll
                 for (int p = 0; p < GR; ++p)
-                    ring_dma<0>(B.rR, sb + (3 * GB + p) * RING_GROUP, v, p * B.pstride);
+                    ring_dma<0>(B.rR, sb + (NBK * GB + p) * RING_GROUP, v, p * B.pstride);
                 // the hand-over slot: lane 0 alone, so 32 bytes land at the slot's end
                 const unsigned es = ein0 + (unsigned)(st * dir * 32);
                 unsigned char *eb = sb + ring_groups<T>() * RING_GROUP;
@@ -1014,6 +1063,7 @@ void lusgs_band_multi(hipStream_t sm, const BandJob *jobs, const int2 *map, int 
 template <typename LT, int NEQ>
 void lusgs_mid(hipStream_t sm, const Dims &d, const LusgsWs &w)
 {
+    if (NEQ == 3 && sizeof(LT) == 4) return;   // scaled form: D^-1 (D y) = y
     long nb = (w.ns + 255) / 256;
     hipLaunchKernelGGL((k_mid<LT, NEQ>), dim3((unsigned)(nb > 4096 ? 4096 : nb)), dim3(256), 0, sm,
                        d, w);
