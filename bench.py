@@ -21,14 +21,15 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per grid point of the forward band sweep of the mean-flow LU-SGS
 # (ilusol3 forward part): diagonal factors, bjm, bkm (9 words each) and rhs (3) read, the
-# forward result (3) written -- 30 + 3 words.  FP32-LHS: the 27 LHS words at 4 bytes and the
-# rhs and result at 4 bytes as well (the sweep runs in FP32).
-BAND_FWD_BYTES = {"fp64": 33 * 8, "fp32": 33 * 4}
+# forward result (3) written -- 30 + 3 words.  FP32-LHS runs the scaled form (bjm and bkm
+# premultiplied by the inverse diagonal, no factors): 18 + 3 words read and 3 written, all at
+# 4 bytes.
+BAND_FWD_BYTES = {"fp64": 33 * 8, "fp32": 24 * 4}
 # HBM bytes per grid point that the same kernel moves, measured at 945x241 with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (scaled by 2.0 and 1.0: the factors that make
 # the two counters read 256 MiB for a plain 256 MiB device copy in the same run)
-BAND_FWD_TRAFFIC = {"fp64": (69080.0 + 8225.0) * 1024 / 227745,
-                    "fp32": (40960.0 + 4207.0) * 1024 / 227745}
+BAND_FWD_TRAFFIC = {"fp64": (69081.0 + 8225.0) * 1024 / 227745,
+                    "fp32": (28733.0 + 4207.0) * 1024 / 227745}
 
 
 def cpu_baseline(case, iters: int) -> dict:
