@@ -95,8 +95,14 @@ k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const dou
                     double acc = lane == 0 ? fxp : lane == 1 ? fyp : lane == 2 ? fxs
                                : lane == 3 ? fys : fmom;
                     if (act) {
+                        if (pass == 0) {
+                            // (acc + (-1)*a) - (+0) == acc - a exactly: one dependent op a face
 #pragma unroll 8
-                        for (int i = 0; i < cnt; ++i) acc = (acc + sgn * A[i]) - B[i];
+                            for (int i = 0; i < cnt; ++i) acc = acc - A[i];
+                        } else {
+#pragma unroll 8
+                            for (int i = 0; i < cnt; ++i) acc = (acc + sgn * A[i]) - B[i];
+                        }
                     }
                     if (lane == 0) fxp = acc;
                     else if (lane == 1) fyp = acc;
