@@ -90,6 +90,7 @@ def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
             t0 = time.perf_counter()
             b.run(steps)                  # ends with every case's stream synchronised
             wall = time.perf_counter() - t0
+            bt = b.timings()
         t0 = time.perf_counter()
         for s in sol:
             s.run(steps)
@@ -107,6 +108,10 @@ def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
             "sequential_value": round(npts * steps / wall_seq / 1e6, 4),
             "speedup_vs_sequential": round(wall_seq / wall, 4), "finite": finite,
             "precision": "fp32-lhs", "tuned_options": tuned,
+            "band_fwd_multi_ms": round(bt["band_fwd_ms"], 5),
+            "band_fwd_multi_achieved_GBs": round(
+                BAND_FWD_BYTES["fp32" if bt["precision"] else "fp64"] * bt["points"]
+                / (bt["band_fwd_ms"] * 1e-3) / 1e9, 3) if bt["band_fwd_ms"] > 0 else None,
             "autotune_seconds": round(t_tune, 2),
             "note": "one stream; the LU-SGS band sweeps of all cases in one launch per "
                     "direction; sequential = the same cases run one after another with smac_run"}

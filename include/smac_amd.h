@@ -129,6 +129,10 @@ smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out);
 /* npass passes of every case; returns after the device has finished them, having checked
  * each case's non-finite and stall flags. */
 smac_status smac_batch_run(smac_batch *b, int npass);
+/* Device time (ms) of the forward mean-flow band sweep launch of the last pass of the last
+ * smac_batch_run (the launch covering the most points: out[1] grid points, precision
+ * out[2]: 0 FP64, 1 FP32-LHS), from HIP events on the batch's stream. */
+smac_status smac_batch_timings(smac_batch *b, double out[3]);
 smac_status smac_batch_destroy(smac_batch *b);
 
 /* Copy a named device array to the host (diagnostics and kernel-level tests):

@@ -652,6 +652,9 @@ struct smac_batch {
     hipStream_t stream = nullptr;                 // band launches; phases fork from and join it
     hipStream_t lane[BATCH_LANES] = {};
     hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_band[2] = {};      // around the forward mean-flow band launch of a run's last pass
+    int band_group = -1;             // the group timed (the one with the most points)
+    double band_ms = 0.0, band_pts = 0.0;
     hipEvent_t ev_join[BATCH_LANES] = {};
     BandGroup sa;                    // SA LU-SGS (neqs 1, FP64) of the LU-SGS cases
     BandGroup mean[2];               // mean-flow LU-SGS: FP64, FP32-LHS
@@ -688,6 +691,8 @@ static void free_batch(smac_batch *b)
         if (b->ev_join[k]) (void)hipEventDestroy(b->ev_join[k]);
     }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    for (auto &e : b->ev_band)
+        if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -786,7 +791,7 @@ static smac_status join_lanes(smac_batch *b)
 
 // one pass of every case, in the order of enqueue_pass for each case.  Case i's kernels of a
 // phase run on lane i % BATCH_LANES; the band sweeps of all cases on the batch stream.
-static smac_status enqueue_batch_pass(smac_batch *b)
+static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
 {
     hipStream_t sm = b->stream;
     auto lane_of = [&](int i) { return b->lane[i % BATCH_LANES]; };
@@ -863,10 +868,13 @@ static smac_status enqueue_batch_pass(smac_batch *b)
     for (int g = 0; g < 2; ++g) {
         BandGroup &gm = b->mean[g];
         if (!gm.nblk) continue;
+        const bool rec = time_band && g == b->band_group;
+        if (rec) HIP_TRY(hipEventRecord(b->ev_band[0], sm));
         if (g == 0)
             lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
         else
             lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+        if (rec) HIP_TRY(hipEventRecord(b->ev_band[1], sm));
         B_TRY(fork_lanes(b));
         for (int i : gm.members) {
             if (g == 0)
@@ -922,7 +930,9 @@ smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out)
         b->cases.push_back(h);
     }
     bool ok = hipStreamCreate(&b->stream) == hipSuccess &&
-              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreate(&b->ev_band[0]) == hipSuccess &&
+              hipEventCreate(&b->ev_band[1]) == hipSuccess;
     for (int k = 0; k < BATCH_LANES && ok; ++k)
         ok = hipStreamCreate(&b->lane[k]) == hipSuccess &&
              hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming) == hipSuccess;
@@ -949,8 +959,19 @@ smac_status smac_batch_run(smac_batch *b, int npass)
         HIP_TRY(hipStreamSynchronize(h->stream_sa));
     }
     B_TRY(check_tables(b));
-    for (int it = 0; it < npass; ++it) B_TRY(enqueue_batch_pass(b));
+    // the mean-flow group with the most points is timed in the last pass
+    double pts[2] = {0.0, 0.0};
+    for (int g = 0; g < 2; ++g)
+        for (int i : b->mean[g].members) pts[g] += (double)b->cases[i]->d.N;
+    b->band_group = pts[0] == 0.0 && pts[1] == 0.0 ? -1 : (pts[1] > pts[0] ? 1 : 0);
+    for (int it = 0; it < npass; ++it) B_TRY(enqueue_batch_pass(b, it == npass - 1));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    if (npass > 0 && b->band_group >= 0) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev_band[0], b->ev_band[1]));
+        b->band_ms = ms;
+        b->band_pts = pts[b->band_group];
+    }
     for (size_t i = 0; i < b->cases.size(); ++i) {
         smac_handle *h = b->cases[i];
         DevStats s;
@@ -962,6 +983,15 @@ smac_status smac_batch_run(smac_batch *b, int npass)
             return fail(SMAC_ERR_NONFINITE, "batch case " + std::to_string(i) +
                                                 ": non-finite at nt=" + std::to_string(h->nt));
     }
+    return SMAC_OK;
+}
+
+smac_status smac_batch_timings(smac_batch *b, double out[3])
+{
+    if (!b || !out) return fail(SMAC_ERR_ARG, "NULL argument");
+    out[0] = b->band_ms;
+    out[1] = b->band_pts;
+    out[2] = b->band_group == 1 ? 1.0 : 0.0;
     return SMAC_OK;
 }
 

@@ -123,6 +123,12 @@ module smac_amd
       integer(c_int), value :: npass
     end function
 
+    integer(c_int) function smac_batch_timings(b, out) bind(C, name="smac_batch_timings")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double), intent(out) :: out(3)
+    end function
+
     integer(c_int) function smac_batch_destroy(b) bind(C, name="smac_batch_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: b

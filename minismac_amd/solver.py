@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsmac_amd
 EXPORTS = [
     "smac_create", "smac_destroy", "smac_set_state", "smac_get_state", "smac_step", "smac_run",
     "smac_get_array", "smac_timings", "smac_set_option", "smac_lrelax", "smac_lusgs",
-    "smac_get_stats", "smac_batch_create", "smac_batch_run", "smac_batch_destroy",
+    "smac_get_stats", "smac_batch_create", "smac_batch_run", "smac_batch_timings",
+    "smac_batch_destroy",
     "smac_last_error", "smac_version",
 ]
 
@@ -60,6 +61,7 @@ def lib():
         L.smac_get_stats.argtypes = [C.c_void_p, P(SmacIterStats)]
         L.smac_batch_create.argtypes = [P(C.c_void_p), C.c_int, P(C.c_void_p)]
         L.smac_batch_run.argtypes = [C.c_void_p, C.c_int]
+        L.smac_batch_timings.argtypes = [C.c_void_p, D]
         L.smac_batch_destroy.argtypes = [C.c_void_p]
         for fn in EXPORTS[:-2]:
             getattr(L, fn).restype = C.c_int
@@ -177,6 +179,13 @@ class Batch:
 
     def run(self, npass: int) -> None:
         _check(lib().smac_batch_run(self.b, int(npass)), "smac_batch_run")
+
+    def timings(self) -> dict:
+        """The forward mean-flow band launch of the last pass: device ms, grid points it
+        covered and its precision (0 FP64, 1 FP32-LHS)."""
+        out = (C.c_double * 3)()
+        _check(lib().smac_batch_timings(self.b, out), "smac_batch_timings")
+        return {"band_fwd_ms": out[0], "points": int(out[1]), "precision": int(out[2])}
 
     def close(self) -> None:
         if getattr(self, "b", None):

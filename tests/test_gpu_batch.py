@@ -105,17 +105,21 @@ def test_batch_rejects_timing_and_duplicates():
 
 
 def test_batch_workload_generator_cases_step():
-    """A few of the batched workload's generated cases (seeded) take finite passes."""
-    cs = batch_cases(64, seed=0)
+    """A few of the batched workload's generated cases (seeded, LU-SGS) take finite passes."""
+    cs = batch_cases(64, seed=0, impsch=2)
     assert len({(c.jmax, c.kmax) for c in cs}) == 3
     pick = [c for c in cs if c.jmax == 239][:2] + [c for c in cs if c.jmax == 477][:1]
     sol = [Solver(c) for c in pick]
     try:
         with Batch(sol) as b:
             b.run(5)
+            t = b.timings()
         for s in sol:
             st = s.stats()
             assert st["nt"] == 5 and np.isfinite(st["rhsnorm"]) and np.isfinite(st["cl"])
+        # the timed launch: the FP64 mean-flow bands of all three cases
+        assert t["precision"] == 0 and t["points"] == sum(c.npts for c in pick)
+        assert 0.0 < t["band_fwd_ms"] < 100.0
     finally:
         for s in sol:
             s.close()
