@@ -452,6 +452,21 @@ SMAC_DEV unsigned long long edge_reload(const unsigned long long *p)
     return v;
 }
 
+// Waits (bounded) until the first word of the slot at p is filled.  Every slot up to the
+// producer band's last step is written and the padding after it is finite, so the wait ends.
+SMAC_DEV void edge_lag(const unsigned long long *p, bool &broken, int *stall)
+{
+    int n = 0;
+    while (edge_reload(p) == EDGE_EMPTY) {
+        if (++n > EDGE_SPIN_MAX) {
+            broken = true;
+            atomicOr(stall, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
 // waits (bounded) until the NEQ slots at p are filled; on time-out raises the stall flag and
 // stops waiting for the rest of the sweep
 template <int NEQ>
@@ -887,6 +902,7 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
                 }
             };
             constexpr int WAIT = vmcnt_imm(ring_m<T>() * ring_vm_per_step<T>());
+            ring_barrier();   // the computing wave has let its producer band get ahead
             for (int st = 0; st < D; ++st) issue(st);
             __builtin_amdgcn_s_waitcnt(WAIT);
             ring_barrier();
@@ -900,6 +916,14 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
             return;
         }
         B.ring = ring_lds;
+        // A band that reads a neighbour's hand-over slots starts RING_D + 4 steps behind it:
+        // the loaders fetch each slot RING_D steps ahead, and a slot fetched before its
+        // producer wrote it costs the computing wave a device-scope round trip to re-read.
+        {
+            constexpr int LAG = ring_d<T>() + 4;
+            edge_lag(B.ein_ptr + (long)(h0 + dir * LAG - B.kin + EPAD) * 4, B.broken, B.stall);
+        }
+        ring_barrier();
         ring_barrier();
         B.ring_read(B.s[0], 0);
         B.ring_read(B.s[1], ring_slot_bytes<T>());
