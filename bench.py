@@ -133,7 +133,7 @@ def main() -> int:
                     help="first copy 256 MiB device to device (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: this benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU decomposition is not part of this library)", file=sys.stderr)
         return 2
 

@@ -192,7 +192,7 @@ static smac_status check_params(const smac_params *p)
     if (p->precision != SMAC_FP64 && p->precision != SMAC_FP32_LHS)
         return fail(SMAC_ERR_ARG, "precision must be SMAC_FP64 or SMAC_FP32_LHS");
     if (p->line_order != SMAC_ORDER_REFERENCE)
-        return fail(SMAC_ERR_ARG, "only SMAC_ORDER_REFERENCE is available in this build");
+        return fail(SMAC_ERR_ARG, "only SMAC_ORDER_REFERENCE is available in this library version");
     if (!(p->dtau > 0.0) || !(p->reynum > 0.0) || !(p->beta > 0.0) || !(p->dt > 0.0))
         return fail(SMAC_ERR_ARG, "dtau, reynum, beta and dt must be positive");
     return SMAC_OK;
