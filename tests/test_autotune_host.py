@@ -45,3 +45,21 @@ def test_store_is_atomic_json(tmp_path):
     autotune._store(str(path), {"k": {"options": {"lr_lines": 16}}})
     assert autotune._load(str(path)) == {"k": {"options": {"lr_lines": 16}}}
     assert autotune._load(str(tmp_path / "absent.json")) == {}
+
+
+def test_batch_case_mix_is_seeded_and_in_range():
+    """The batched workload's case mix: reproducible for a seed, parameters inside their
+    ranges, sizes from the three grids."""
+    import numpy as np
+    from minismac_amd.case import BATCH_SIZES, batch_cases
+    a = batch_cases(8, seed=3)
+    b = batch_cases(8, seed=3)
+    c = batch_cases(8, seed=4)
+    assert [x.name for x in a] == [x.name for x in b]
+    assert [x.name for x in a] != [x.name for x in c]
+    sizes = {(jm, km) for jm, km, _ in BATCH_SIZES}
+    for x, y in zip(a, b):
+        assert (x.jmax, x.kmax) in sizes
+        assert -4.0 <= x.params.alpha <= 16.0
+        assert 1e5 <= x.params.reynum <= 1e7
+        assert np.array_equal(x.x, y.x) and np.array_equal(x.y, y.y)
