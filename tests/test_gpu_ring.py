@@ -1,7 +1,7 @@
 """The LDS-ring form of the 3x3 LU-SGS band sweeps (option lusgs_ring): operands streamed into
 LDS by loader wavefronts.  It performs the same per-point arithmetic in the same order as the
-register-prefetch form, which the oracle pins, so the results must be bit-identical to it,
-for FP64 and FP32-LHS, one band and several, alone and in a batch."""
+register-prefetch form (lusgs_ring=0), so the results must be bit-identical to it, for FP64
+and FP32-LHS, one band and several, alone and in a batch."""
 import numpy as np
 import pytest
 
@@ -26,7 +26,7 @@ def test_ring_sweeps_match_register_sweeps(precision, shape, rows):
     case = (config_case(1, impsch=2, precision=precision) if jm == 945 else
             naca_case(jm, km, alpha=5.0, reynum=1e6, beta=10.0, impsch=2, precision=precision))
     print(f"ring vs registers: {jm}x{km} rows={rows} precision={precision}", flush=True)
-    q0, a0, v0, s0 = _run(case, {"lusgs_rows": rows}, 6)
+    q0, a0, v0, s0 = _run(case, {"lusgs_rows": rows, "lusgs_ring": 0}, 6)
     q1, a1, v1, s1 = _run(case, {"lusgs_rows": rows, "lusgs_ring": 1}, 6)
     assert np.array_equal(q0, q1) and np.array_equal(a0, a1) and np.array_equal(v0, v1)
     assert s0["rhsnorm"] == s1["rhsnorm"] and s0["cl"] == s1["cl"]
@@ -38,11 +38,35 @@ def test_ring_in_a_batch_matches_register_sweeps_alone():
                        precision=SMAC_FP32_LHS),
              naca_case(239, 61, m=0.05, alpha=-2.0, reynum=3e6, beta=10.0, impsch=2,
                        precision=SMAC_FP32_LHS)]
-    ref = [_run(c, {}, 5) for c in cases]
+    ref = [_run(c, {"lusgs_ring": 0}, 5) for c in cases]
     sol = [Solver(c, options={"lusgs_ring": 1}) for c in cases]
     try:
         with Batch(sol) as b:
             b.run(5)
+        for (q0, a0, v0, s0), s in zip(ref, sol):
+            q1, a1, v1 = s.get_state()
+            assert np.array_equal(q0, q1) and np.array_equal(a0, a1) and np.array_equal(v0, v1)
+    finally:
+        for s in sol:
+            s.close()
+
+
+@pytest.mark.parametrize("first_ring", [0, 1])
+def test_batch_register_launch_matches_ring_alone(first_ring):
+    """A multi-case band launch takes its sweep form from its first case.  With that case on
+    the register-prefetch form (lusgs_ring=0) the FP32-LHS batch runs k_band_multi, with it on
+    the ring form k_band_ring_multi; either way every case equals its ring-form run alone."""
+    cases = [naca_case(945, 241, m=0.04, alpha=8.0, reynum=3e6, beta=10.0, impsch=2,
+                       wall_spacing=2e-6, precision=SMAC_FP32_LHS),
+             naca_case(239, 61, m=0.02, alpha=4.0, reynum=1e6, beta=10.0, impsch=2,
+                       precision=SMAC_FP32_LHS)]
+    print(f"batch with first case lusgs_ring={first_ring}", flush=True)
+    ref = [_run(c, {"lusgs_ring": 1}, 4) for c in cases]
+    sol = [Solver(cases[0], options={"lusgs_ring": first_ring}),
+           Solver(cases[1], options={"lusgs_ring": 1})]
+    try:
+        with Batch(sol) as b:
+            b.run(4)
         for (q0, a0, v0, s0), s in zip(ref, sol):
             q1, a1, v1 = s.get_state()
             assert np.array_equal(q0, q1) and np.array_equal(a0, a1) and np.array_equal(v0, v1)
