@@ -180,6 +180,29 @@ smac_status smac_lusgs(const smac_params *p, int jmax, int kmax, int neqs, int n
                        const double *btc, const double *bjm, const double *bjp,
                        const double *bkm, const double *bkp, double *scr1);
 
+/* ---- Allocation-free LU-SGS ------------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc.  The solve of
+ * smac_lusgs (the same kernels, bit-identical results) on arrays that are already in device
+ * memory, enqueued on `stream` (a hipStream_t; NULL: the null stream) and not waited for.
+ * It calls no hipMalloc/hipFree: its workspace (the sheared planes and the band hand-over
+ * slots) lives in one static device array of SMAC_NOALLOC_SCRATCH_BYTES on each GPU it runs
+ * on, held until the process ends.  A grid whose workspace does not fit is rejected with
+ * SMAC_ERR_SIZE (neqs = 3: up to about 477x121 points).
+ *  - q, dq, rhs, btc, bjm, bjp, bkm, bkp, scr1: device arrays, the layouts of smac_lusgs.
+ *  - wake: npair (jt, jb) pairs, 1-based, in device memory (a pair outside 1..jmax is skipped).
+ *  - stall: NULL, or a device int set to 0 by the call and to 1 if a band hand-over timed out.
+ *  - As for smac_lusgs, bjm(1,k) and bjp(jmax,k) must be zero (the C-grid rows); device
+ *    arrays are not inspected on the host, so this is the caller's to ensure.
+ * Every call on a GPU shares that GPU's workspace: calls that overlap on one GPU race, so
+ * issue them on one stream (or order them with events). */
+#define SMAC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+smac_status smac_lusgs_noalloc(const smac_params *p, int jmax, int kmax, int neqs, int npair,
+                               const int *wake, const double *q, double *dq, double *rhs,
+                               const double *btc, const double *bjm, const double *bjp,
+                               const double *bkm, const double *bkp, double *scr1, int *stall,
+                               void *stream);
+
 /* Text of the last error on this thread ("" if none). */
 const char *smac_last_error(void);
 /* Library version string. */

@@ -22,6 +22,9 @@ static smac_status fail(smac_status st, const std::string &msg)
     return st;
 }
 
+// sets the calling thread's error text for the entry points of other translation units
+smac_status smac_fail_text(smac_status st, const char *msg) { return fail(st, msg); }
+
 #define HIP_TRY(expr)                                                                         \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \

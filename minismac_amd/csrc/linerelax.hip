@@ -329,14 +329,16 @@ k_sweep_ref(Dims d, int eta, int nsweep, double underr, double *__restrict__ dq,
     }
 }
 
-// wake-cut injection, bcimpds single-rank branch (ref/bc.f:571-591): k = 0 row of the cut
-__global__ void k_wake(long N, int jm, int neqs, int npair, const int *__restrict__ pairs,
-                       const double *__restrict__ sq, double *__restrict__ ds,
-                       double *__restrict__ rhs)
+// wake-cut injection, bcimpds single-rank branch (ref/bc.f:571-591): k = 0 row of the cut.
+// Pairs whose indices fall outside the row are skipped.
+__global__ void k_wake(long N, int jm, int neqs, int npair, int base,
+                       const int *__restrict__ pairs, const double *__restrict__ sq,
+                       double *__restrict__ ds, double *__restrict__ rhs)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npair) return;
-    const long jt = pairs[2 * i], jb = pairs[2 * i + 1];
+    const long jt = pairs[2 * i] - base, jb = pairs[2 * i + 1] - base;
+    if (jt < 0 || jt >= jm || jb < 0 || jb >= jm) return;
     for (int n = 0; n < neqs; ++n) {
         double snew = 0.5 * (sq[n * N + jt + jm] + sq[n * N + jb + jm]);
         double dsnew = 0.5 * (ds[n * N + jt + jm] + ds[n * N + jb + jm]);
@@ -357,7 +359,7 @@ void launch_wake(hipStream_t sm, const Dims &d, int neqs, const WakePairs &w, co
 {
     if (w.n <= 0) return;
     hipLaunchKernelGGL(k_wake, dim3((w.n + 63) / 64), dim3(64), 0, sm, d.N, d.jm, neqs, w.n,
-                       w.pairs, sq, ds, rhs);
+                       w.base, w.pairs, sq, ds, rhs);
 }
 
 template <typename LT>

@@ -21,7 +21,8 @@ struct Scratch {
 
 struct WakePairs {
     int n;
-    const int *pairs;   // device, 2n ints, 0-based (jt, jb)
+    const int *pairs;   // device, 2n ints (jt, jb)
+    int base = 0;       // index base of `pairs` (0, or 1 for caller-owned Fortran indices)
 };
 
 struct Walls {

@@ -23,7 +23,7 @@ EXPORTS = [
     "smac_create", "smac_destroy", "smac_set_state", "smac_get_state", "smac_step", "smac_run",
     "smac_get_array", "smac_timings", "smac_set_option", "smac_lrelax", "smac_lusgs",
     "smac_get_stats", "smac_batch_create", "smac_batch_run", "smac_batch_timings",
-    "smac_batch_destroy",
+    "smac_batch_destroy", "smac_lusgs_noalloc",
     "smac_last_error", "smac_version",
 ]
 
@@ -63,6 +63,8 @@ def lib():
         L.smac_batch_run.argtypes = [C.c_void_p, C.c_int]
         L.smac_batch_timings.argtypes = [C.c_void_p, D]
         L.smac_batch_destroy.argtypes = [C.c_void_p]
+        L.smac_lusgs_noalloc.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_void_p] + [C.c_void_p] * 11
         for fn in EXPORTS[:-2]:
             getattr(L, fn).restype = C.c_int
         L.smac_last_error.restype = C.c_char_p
@@ -234,3 +236,15 @@ def lusgs(params: SmacParams, neqs: int, q, dq, rhs, btc, bjm, bjp, bkm, bkp, sc
                             dptr(rhs), dptr(btc), dptr(bjm), dptr(bjp), dptr(bkm), dptr(bkp),
                             dptr(scr1)), "smac_lusgs")
     del keep
+
+
+def lusgs_noalloc(params: SmacParams, jmax: int, kmax: int, neqs: int, npair: int, wake_ptr: int,
+                  q_ptr: int, dq_ptr: int, rhs_ptr: int, btc_ptr: int, bjm_ptr: int, bjp_ptr: int,
+                  bkm_ptr: int, bkp_ptr: int, scr1_ptr: int, stall_ptr: int = 0,
+                  stream: int = 0) -> None:
+    """smac_lusgs_noalloc on device addresses (e.g. torch tensors' data_ptr()): enqueued on
+    `stream` and not waited for."""
+    _check(lib().smac_lusgs_noalloc(C.byref(params), jmax, kmax, neqs, npair, wake_ptr or None,
+                                    q_ptr, dq_ptr, rhs_ptr, btc_ptr, bjm_ptr, bjp_ptr, bkm_ptr,
+                                    bkp_ptr, scr1_ptr, stall_ptr or None, stream or None),
+           "smac_lusgs_noalloc")
