@@ -17,45 +17,64 @@ from minismac_amd.case import Case, dptr, fortran
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 LIB_PATH = os.path.join(REF_DIR, "libsmac_oracle.so")
+SLIPS_PATH = os.path.join(REF_DIR, "libsmac_oracle_slips.so")
 
 _lib = None
+_slips = None
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise FileNotFoundError(f"{LIB_PATH} missing: run `make -C oracle oracle`")
-        L = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        L.orc_create.restype = C.c_void_p
-        L.orc_create.argtypes = [P(SmacParams), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
-                                 P(C.c_double), P(C.c_double), C.c_int, C.c_void_p]
-        L.orc_destroy.argtypes = [C.c_void_p]
-        for fn in ("orc_step", "orc_assemble", "orc_finish"):
-            getattr(L, fn).argtypes = [C.c_void_p, P(SmacIterStats)]
-            getattr(L, fn).restype = C.c_int
-        L.orc_sa_assemble.argtypes = [C.c_void_p]
-        L.orc_sa_assemble.restype = C.c_int
-        L.orc_solve.argtypes = [C.c_void_p]
-        L.orc_solve.restype = C.c_int
-        L.orc_array.argtypes = [C.c_void_p, C.c_char_p]
-        L.orc_array.restype = P(C.c_double)
-        L.orc_set_counters.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
-        L.orc_metric.argtypes = [C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
-                                 P(C.c_double)]
-        L.orc_npairs.argtypes = [C.c_void_p]
-        L.orc_npairs.restype = C.c_int
-        L.orc_pairs.argtypes = [C.c_void_p]
-        L.orc_pairs.restype = P(C.c_int)
-        L.orc_lrelax.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, P(C.c_int)] + \
-            [P(C.c_double)] * 8
-        L.orc_lrelax.restype = C.c_int
-        L.orc_lusgs.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_int)] + \
-            [P(C.c_double)] * 9
-        L.orc_lusgs.restype = C.c_int
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def slips_lib():
+    """The oracle built with planted slips (negative controls of the pinning tests):
+    ``slips_lib().orc_set_slip(n)`` turns slip n on, 0 turns them all off."""
+    global _slips
+    if _slips is None:
+        _slips = _load(SLIPS_PATH)
+        _slips.orc_set_slip.argtypes = [C.c_int]
+        _slips.orc_set_slip.restype = None
+    return _slips
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: run `make -C oracle oracle`")
+    L = C.CDLL(path)
+    P = C.POINTER
+    L.orc_create.restype = C.c_void_p
+    L.orc_create.argtypes = [P(SmacParams), C.c_int, C.c_int, P(C.c_double), P(C.c_double),
+                             P(C.c_double), P(C.c_double), C.c_int, C.c_void_p]
+    L.orc_destroy.argtypes = [C.c_void_p]
+    for fn in ("orc_step", "orc_assemble", "orc_finish"):
+        getattr(L, fn).argtypes = [C.c_void_p, P(SmacIterStats)]
+        getattr(L, fn).restype = C.c_int
+    L.orc_sa_assemble.argtypes = [C.c_void_p]
+    L.orc_sa_assemble.restype = C.c_int
+    L.orc_solve.argtypes = [C.c_void_p]
+    L.orc_solve.restype = C.c_int
+    L.orc_array.argtypes = [C.c_void_p, C.c_char_p]
+    L.orc_array.restype = P(C.c_double)
+    L.orc_set_counters.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+    L.orc_metric.argtypes = [C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
+                             P(C.c_double)]
+    L.orc_npairs.argtypes = [C.c_void_p]
+    L.orc_npairs.restype = C.c_int
+    L.orc_pairs.argtypes = [C.c_void_p]
+    L.orc_pairs.restype = P(C.c_int)
+    L.orc_lrelax.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, P(C.c_int)] + \
+        [P(C.c_double)] * 8
+    L.orc_lrelax.restype = C.c_int
+    L.orc_lusgs.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_int)] + \
+        [P(C.c_double)] * 9
+    L.orc_lusgs.restype = C.c_int
+    L.orc_face_split.argtypes = [C.c_double] * 6 + [P(C.c_double)] * 2
+    L.orc_face_split.restype = None
+    return L
 
 
 _PLANES = {"q": 3, "dq": 3, "s": 3, "anut": 1, "vnut": 1, "rtxy": 6, "dj": 1, "x": 1, "y": 1,
@@ -66,8 +85,9 @@ _PLANES = {"q": 3, "dq": 3, "s": 3, "anut": 1, "vnut": 1, "rtxy": 6, "dj": 1, "x
 class Oracle:
     """One case in the C restatement (CPU).  Arrays come back as Fortran-ordered copies."""
 
-    def __init__(self, case: Case):
-        L = lib()
+    def __init__(self, case: Case, L=None):
+        L = L or lib()
+        self.L = L
         self.case = case
         self.jmax, self.kmax = case.jmax, case.kmax
         self._x, self._y = fortran(case.x), fortran(case.y)
@@ -84,7 +104,7 @@ class Oracle:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self.L.orc_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -97,27 +117,27 @@ class Oracle:
         return st.as_dict()
 
     def step(self) -> dict:
-        return self._call(lib().orc_step)
+        return self._call(self.L.orc_step)
 
     def assemble(self) -> dict:
-        return self._call(lib().orc_assemble)
+        return self._call(self.L.orc_assemble)
 
     def sa_assemble(self) -> None:
-        lib().orc_sa_assemble(self.h)
+        self.L.orc_sa_assemble(self.h)
 
     def solve(self) -> None:
-        rc = lib().orc_solve(self.h)
+        rc = self.L.orc_solve(self.h)
         if rc:
             raise RuntimeError(f"oracle solve failed with status {rc}")
 
     def finish(self) -> dict:
-        return self._call(lib().orc_finish)
+        return self._call(self.L.orc_finish)
 
     def array(self, name: str) -> np.ndarray:
         planes = 9 if name in ("btc", "bjm", "bjp", "bkm", "bkp") or name.startswith("lhs_") \
             else _PLANES[name]
         n = self.jmax * self.kmax * planes
-        p = lib().orc_array(self.h, name.encode())
+        p = self.L.orc_array(self.h, name.encode())
         if not p:
             raise KeyError(name)
         a = np.ctypeslib.as_array(p, shape=(n,)).copy()
@@ -127,17 +147,25 @@ class Oracle:
     def set_array(self, name: str, values: np.ndarray) -> None:
         planes = _PLANES.get(name, 9)
         n = self.jmax * self.kmax * planes
-        p = lib().orc_array(self.h, name.encode())
+        p = self.L.orc_array(self.h, name.encode())
         dst = np.ctypeslib.as_array(p, shape=(n,))
         dst[:] = np.asarray(values, dtype=np.float64).ravel(order="F")
 
     def set_counters(self, nt: int, resmax0: float, clift: float) -> None:
-        lib().orc_set_counters(self.h, nt, resmax0, clift)
+        self.L.orc_set_counters(self.h, nt, resmax0, clift)
 
     def wake_pairs(self) -> np.ndarray:
-        n = lib().orc_npairs(self.h)
-        p = lib().orc_pairs(self.h)
+        n = self.L.orc_npairs(self.h)
+        p = self.L.orc_pairs(self.h)
         return np.array([[p[2 * i], p[2 * i + 1]] for i in range(n)], dtype=np.int32)
+
+
+def face_split(rt, rx, ry, q2, q3, beta, L=None):
+    """A+ and A- of one face (orc_face_split) as 3x3 arrays."""
+    ap = np.zeros(9)
+    am = np.zeros(9)
+    (L or lib()).orc_face_split(rt, rx, ry, q2, q3, beta, dptr(ap), dptr(am))
+    return ap.reshape(3, 3, order="F"), am.reshape(3, 3, order="F")
 
 
 def metric(x: np.ndarray, y: np.ndarray):

@@ -27,6 +27,32 @@
 
 #define MAXREG 64
 
+/* Planted slips, for the negative controls of tests/test_oracle_pinning.py only: built with
+ * -DORC_SLIPS into a separate library (_ref/libsmac_oracle_slips.so), where orc_set_slip(n)
+ * turns on one plausible transcription mistake.  In the oracle itself SLIP(n) is the
+ * constant 0 and every slip site compiles to the reference's expression. */
+#ifdef ORC_SLIPS
+static int g_slip = 0;
+void orc_set_slip(int n) { g_slip = n; }
+#define SLIP(n) (g_slip == (n))
+#else
+#define SLIP(n) 0
+#endif
+enum {
+    SLIP_FACE_X21 = 1,    /* face_split: sign of x21                                    */
+    SLIP_FACE_XI13 = 2,   /* face_split: xi13 built from q3 instead of q2                */
+    SLIP_FLUX_R6 = 3,     /* flux3: r6 = 1/3                                             */
+    SLIP_FLUX_DFM = 4,    /* flux3: sign of the A- differences in the third-order term   */
+    SLIP_FLUX_ENDS = 5,   /* flux3: the two one-sided end forms exchanged                */
+    SLIP_VISC_UV = 6,     /* viscterms: u equation fed with the v difference (xi)        */
+    SLIP_VISC_RY = 7,     /* viscterms: ry^2 missing from the xi face coefficient        */
+    SLIP_BC25_VORTEX = 8, /* far field: point-vortex velocity of the wrong sense         */
+    SLIP_BC25_CHAR = 9,   /* far field: pressure row along the U-c characteristic        */
+    SLIP_SA_FV2 = 10,     /* tsafill: fv2 = (1+chi/cv2)^-2                               */
+    SLIP_SA_LIN = 11,     /* tsafill: linearisation without the fw derivative            */
+    SLIP_FORCE_NORMAL = 12 /* force: x component of the wall normal taken from eta_y     */
+};
+
 struct orc_case {
     smac_params p;
     double vnu;
@@ -147,14 +173,14 @@ static void face_split(double rt, double rx, double ry, double q2, double q3, do
     double eig = cvel + rt;
     double eig1 = 0.5 * (eig + fabs(eig));
     double eig2 = cvel + rt * 0.5 + cc;
-    double x21 = -ry * eig1 / (cc * cc - 0.25 * rt * rt);
+    double x21 = (SLIP(SLIP_FACE_X21) ? ry : -ry) * eig1 / (cc * cc - 0.25 * rt * rt);
     double x31 = rx * eig1 / (cc * cc - 0.25 * rt * rt);
     double x12 = 0.5 * eig2 / cc;
     double x22 = 0.5 * eig2 * (q2 * eig2 + beta * rx) / (beta * cc * (cc - 0.5 * rt));
     double x32 = 0.5 * eig2 * (q3 * eig2 + beta * ry) / (beta * cc * (cc - 0.5 * rt));
     double xi11 = ry * q2 - rx * q3;
     double xi12 = -q3 * (cvel + rt) - ry * beta;
-    double xi13 = q2 * (cvel + rt) + rx * beta;
+    double xi13 = (SLIP(SLIP_FACE_XI13) ? q3 : q2) * (cvel + rt) + rx * beta;
     double xi21 = cc - cvel - 0.5 * rt;
     double xi22 = rx * beta;
     double xi23 = ry * beta;
@@ -176,6 +202,18 @@ static void face_split(double rt, double rx, double ry, double q2, double q3, do
     am[0][2] = -ap[0][2] + ry * beta;
     am[1][2] = -ap[1][2] + ry * q2;
     am[2][2] = -ap[2][2] + rt + rx * q2 + 2. * ry * q3;
+}
+
+void orc_face_split(double rt, double rx, double ry, double q2, double q3, double beta,
+                    double *ap, double *am)
+{
+    double a[3][3], b[3][3];
+    face_split(rt, rx, ry, q2, q3, beta, a, b);
+    for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n) {
+            ap[m + 3 * n] = a[m][n];
+            am[m + 3 * n] = b[m][n];
+        }
 }
 
 static void point_jac(double rt, double rx, double ry, double u, double v, double beta,
@@ -223,7 +261,8 @@ static void flux3_line(orc_case *c, int n, const double *pq, const double *uq, c
             DFM(i, m) = am[i - 1][m - 1][0] * dq1 + am[i - 1][m - 1][1] * dq2 + am[i - 1][m - 1][2] * dq3;
         }
     }
-    const double r6 = (double)(1.0f / 6.0f); /* r6 = 1./6. is a default-REAL quotient */
+    const double r6 = SLIP(SLIP_FLUX_R6) ? 1.0 / 3.0
+                                         : (double)(1.0f / 6.0f); /* r6 = 1./6.: a default-REAL quotient */
     for (int i = 1; i <= n - 1; ++i) {
         int im = i - 1 > 1 ? i - 1 : 1;
         int ip = i + 1 < n ? i + 1 : n;
@@ -233,7 +272,13 @@ static void flux3_line(orc_case *c, int n, const double *pq, const double *uq, c
         for (int m = 1; m <= 3; ++m) {
             double fm = (DFP(i, m) + DFM(i, m) - DFP(ip, m) - DFM(ip, m)) * r6;
             double fp = (DFP(im, m) + DFM(im, m) - DFP(i, m) - DFM(i, m)) * r6;
-            double f3 = (DFP(im, m) - DFP(i, m) + DFM(i, m) - DFM(ip, m)) * r6;
+            double f3 = SLIP(SLIP_FLUX_DFM) ? (DFP(im, m) - DFP(i, m) - DFM(i, m) + DFM(ip, m)) * r6
+                                            : (DFP(im, m) - DFP(i, m) + DFM(i, m) - DFM(ip, m)) * r6;
+            if (SLIP(SLIP_FLUX_ENDS)) {
+                double t = fm;
+                fm = fp;
+                fp = t;
+            }
             FF(i, m) = 0.5 * (FV(ip, m) + FV(i, m)) +
                        cdis * ((double)(cp * cm) * f3 + (double)(cm * (1.f - cp)) * fp +
                                (double)(cp * (1.f - cm)) * fm);
@@ -343,7 +388,7 @@ static void viscterms(orc_case *c)
             vdif[j - 1] = (float)(Q3(q, j + 1, k, 3) - Q3(q, j, k, 3));
             double rx = 0.5 * (RT(j, k, 1, 2) + RT(j + 1, k, 1, 2));
             double ry = 0.5 * (RT(j, k, 1, 3) + RT(j + 1, k, 1, 3));
-            vf[j - 1] = (float)(djv * (rx * rx + ry * ry));
+            vf[j - 1] = SLIP(SLIP_VISC_RY) ? (float)(djv * (rx * rx)) : (float)(djv * (rx * rx + ry * ry));
         }
         for (int j = 2; j <= jm - 1; ++j) {
             double a = vf[j - 2], b = vf[j - 1];
@@ -353,7 +398,8 @@ static void viscterms(orc_case *c)
             B9(c->btc, j, k, 3, 3) = B9(c->btc, j, k, 3, 3) + a + b;
             B9(c->bjp, j, k, 2, 2) = B9(c->bjp, j, k, 2, 2) - b;
             B9(c->bjp, j, k, 3, 3) = B9(c->bjp, j, k, 3, 3) - b;
-            float t2 = vf[j - 1] * udif[j - 1] - vf[j - 2] * udif[j - 2];
+            float t2 = SLIP(SLIP_VISC_UV) ? vf[j - 1] * vdif[j - 1] - vf[j - 2] * vdif[j - 2]
+                                          : vf[j - 1] * udif[j - 1] - vf[j - 2] * udif[j - 2];
             float t3 = vf[j - 1] * vdif[j - 1] - vf[j - 2] * vdif[j - 2];
             Q3(s, j, k, 2) = Q3(s, j, k, 2) + (double)t2;
             Q3(s, j, k, 3) = Q3(s, j, k, 3) + (double)t3;
@@ -421,6 +467,10 @@ static int bccgridk(orc_case *c, int ibval, int k, int jbeg, int jend)
         double dy = S2(y, j, k);
         double ucirc = clift * dy / (4. * pi * (dx * dx + dy * dy));
         double vcirc = -clift * dx / (4. * pi * (dx * dx + dy * dy));
+        if (SLIP(SLIP_BC25_VORTEX)) {
+            ucirc = -ucirc;
+            vcirc = -vcirc;
+        }
         B9(c->btc, j, k, 2, 2) = 1.0;
         B9(c->btc, j, k, 3, 3) = 1.0;
         Q3(s, j, k, 2) = uinf + ucirc - Q3(q, j, k, 2);
@@ -449,6 +499,7 @@ static int bccgridk(orc_case *c, int ibval, int k, int jbeg, int jend)
             double r2 = RT(j, k, jk, 2), r3 = RT(j, k, jk, 3);
             double qq = r2 * Q3(q, j, k, 2) + r3 * Q3(q, j, k, 3);
             double cc = sqrt(qq * qq + beta * (r2 * r2 + r3 * r3));
+            if (SLIP(SLIP_BC25_CHAR)) cc = -cc;
             B9(c->bkm, j, k, 1, 1) = -(r2 * r2 + r3 * r3);
             B9(c->bkm, j, k, 1, 2) = -r2 * (qq + cc);
             B9(c->bkm, j, k, 1, 3) = -r3 * (qq + cc);
@@ -1219,7 +1270,7 @@ static void tsafill(orc_case *c)
             double ss = S2(c->vort, j, k);
             double chi = dmax(an, epss2);
             double fv1 = pw_sq(chi, 3) / (pw_sq(chi, 3) + pw_sq(cv1, 3));
-            double fv2 = 1 / pw_sq(1. + chi / cv2, 3);
+            double fv2 = SLIP(SLIP_SA_FV2) ? 1 / pw_sq(1. + chi / cv2, 2) : 1 / pw_sq(1. + chi / cv2, 3);
             double fv3 = (1 + chi * fv1) / chi * (1 - fv2);
             double ft2 = ct3tmp * exp(-(ct4 * pw_sq(chi, 2)));
             double sst = fv3 * ss + vnu * chi * fv2 * rkd2;
@@ -1239,7 +1290,7 @@ static void tsafill(orc_case *c)
             double sstp = fv3p * ss + vnu * (chi * fv2p + fv2) * rkd2;
             double rrp = vnu * rkd2 * (sst - chi * sstp) / pw_sq(sst, 2);
             double ggp = rrp * (1. + cw2 * (6. * pw_sq(rr, 5) - 1.));
-            double fwp = ggp * fwdg * (1. - gg6 / (gg6 + cw3r6));
+            double fwp = SLIP(SLIP_SA_LIN) ? 0.0 : ggp * fwdg * (1. - gg6 / (gg6 + cw3r6));
             double ppp = cb1 * (sstp - ft2p * sst - ft2 * sstp);
             double ddp = vnu * (cw1 * fw - cb1 * ft2 / pw_sq(akarman, 2) +
                                 (cw1 * fwp - cb1 * ft2p / pw_sq(akarman, 2)) * chi) / pw_sq(sminlim, 2);
@@ -1535,7 +1586,8 @@ int orc_finish(orc_case *c, smac_iter_stats *st)
         fxp = 0.; fyp = 0.; fxs = 0.; fys = 0.;
         for (int j = jstart; j <= jend - 1; ++j) {
             double pave = 0.5 * (Q3(q, j + 1, k, 1) + Q3(q, j, k, 1));
-            double da1 = 0.5 * (RT(j + 1, k, 2, 2) + RT(j, k, 2, 2));
+            double da1 = SLIP(SLIP_FORCE_NORMAL) ? 0.5 * (RT(j + 1, k, 2, 3) + RT(j, k, 2, 3))
+                                                 : 0.5 * (RT(j + 1, k, 2, 2) + RT(j, k, 2, 2));
             double da2 = 0.5 * (RT(j + 1, k, 2, 3) + RT(j, k, 2, 3));
             double xave = 0.5 * (S2(x, j + 1, k) + S2(x, j, k)) - x0;
             double yave = 0.5 * (S2(y, j + 1, k) + S2(y, j, k)) - y0;

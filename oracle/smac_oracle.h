@@ -9,9 +9,12 @@
  * MPI), so the restatement is pinned by tests/test_oracle_pinning.py against what the
  * mathematics fixes (dense solves of the factorised systems, factor identities, the
  * wake-cut mean, freestream preservation and the block-sum identity, brute-force wall
- * distance, first-index argmax) and by tests/test_grid_metric.py (exact metrics of affine
- * grids).  Parity unpinned: the SA source assembly (tsafill), the far-field point-vortex
- * rows and the wall normals of the pressure force.
+ * distance, first-index argmax), by tests/test_oracle_closed_forms.py (the face split as the
+ * positive part of the flux Jacobian, third-order flux convergence with second-order ends,
+ * closed-form viscous terms, the far-field vortex and characteristic rows, the SA source in
+ * closed form and its linearisation against differences, second-order convergence of the
+ * wall force), each with a planted-slip negative control, and by tests/test_grid_metric.py
+ * (exact metrics of affine grids).
  *
  * Layout: the reference's Fortran arrays (j fastest; see include/smac_amd.h).
  */
@@ -49,6 +52,11 @@ int orc_finish(orc_case *c, smac_iter_stats *st);
 double *orc_array(orc_case *c, const char *name);
 /* Set nt, resmax0, clift (restart from a saved state). */
 void orc_set_counters(orc_case *c, int nt, double resmax0, double clift);
+
+/* The face split of conup on its own (ref/conup.f:127-185): A+ and A- of one face,
+ * entry (m,n) at m + 3n. */
+void orc_face_split(double rt, double rx, double ry, double q2, double q3, double beta,
+                    double *ap, double *am);
 
 /* ref/metric.f:49-278 on its own. */
 void orc_metric(int jmax, int kmax, const double *x, const double *y, double *rtxy, double *dj);

@@ -21,9 +21,8 @@ index, or a transposed block fails:
   force/fmk            no motion: zero shear force; uniform pressure: force and moment
                        exactly linear in p (the discrete normals themselves are not pinned)
   isamax               ties resolve to the first index in Fortran order
-Not pinned here (parity unpinned): the SA source assembly (tsafill), the far-field
-point-vortex rows (bc 25) and the wall-normal discretisation of the pressure force have no
-closed form that the checks above reach.
+The face split, the third-order flux and its end forms, the viscous terms, the far-field
+rows, the SA source and the wall force are pinned in test_oracle_closed_forms.py.
 """
 import numpy as np
 import pytest
