@@ -12,8 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libsmac_amd.so")
-SOURCES = ["api.hip", "assemble.hip", "linerelax.hip", "lusgs.hip", "sa.hip", "forces.hip",
-           "noalloc.hip"]
+SOURCES = ["api.hip", "assemble.hip", "linerelax.hip", "linescan.hip", "lusgs.hip", "sa.hip",
+           "forces.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: the kernels keep the reference's operation order (no FMA contraction),

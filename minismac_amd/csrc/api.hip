@@ -58,6 +58,10 @@ struct smac_handle {
     int lusgs_rows = 64;
     int lusgs_ring = 1;    // the 3x3 sweeps stream their operands through an LDS ring
     int lr_lines = 64;     // lines per workgroup of the line-relaxation factor kernels
+    // line relaxation with each line solved by prefix scans inside the line (1), or by the
+    // reference's sequential substitution (0, bit-identical to the reference)
+    int lr_solve = 0;
+    LrScan lrs = {};
     std::vector<void *> lws_allocs;
     Walls walls = {};
     std::vector<BcRegion> regs;
@@ -194,8 +198,10 @@ static smac_status check_params(const smac_params *p)
         return fail(SMAC_ERR_ARG, "ntksp > 0 (SA eta sweeps) is not supported; the reference uses 0");
     if (p->precision != SMAC_FP64 && p->precision != SMAC_FP32_LHS)
         return fail(SMAC_ERR_ARG, "precision must be SMAC_FP64 or SMAC_FP32_LHS");
-    if (p->line_order != SMAC_ORDER_REFERENCE)
-        return fail(SMAC_ERR_ARG, "only SMAC_ORDER_REFERENCE is available in this library version");
+    if (p->line_order != SMAC_ORDER_REFERENCE && p->line_order != SMAC_ORDER_ZEBRA)
+        return fail(SMAC_ERR_ARG, "line_order must be SMAC_ORDER_REFERENCE or SMAC_ORDER_ZEBRA");
+    if (p->line_order == SMAC_ORDER_ZEBRA && p->impsch != 1)
+        return fail(SMAC_ERR_ARG, "SMAC_ORDER_ZEBRA orders the lines of line relaxation (impsch = 1)");
     if (!(p->dtau > 0.0) || !(p->reynum > 0.0) || !(p->beta > 0.0) || !(p->dt > 0.0))
         return fail(SMAC_ERR_ARG, "dtau, reynum, beta and dt must be positive");
     return SMAC_OK;
@@ -215,8 +221,9 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
     *out = nullptr;
     smac_status rc = check_params(p);
     if (rc) return rc;
-    if (p->precision == SMAC_FP32_LHS && p->impsch != 2)
-        return fail(SMAC_ERR_ARG, "SMAC_FP32_LHS is available with LU-SGS (impsch = 2)");
+    // FP32-LHS line relaxation and the zebra order exist only in the scan form of the solve
+    const bool scan_only = p->impsch == 1 && (p->precision == SMAC_FP32_LHS ||
+                                              p->line_order == SMAC_ORDER_ZEBRA);
     if (!x || !y || !rtxy || !dj || !regs || nreg < 1)
         return fail(SMAC_ERR_ARG, "grid, metric or BC arrays missing");
     if (jmax < 5 || kmax < 5 || jmax > 8100 || kmax > 8100)
@@ -224,6 +231,8 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
     if ((p->impsch == 1 && (jmax > 2700 || kmax > 2700)) || (p->iturb && jmax > 2000))
         return fail(SMAC_ERR_SIZE, "reference-order line sweeps stage a line in LDS: "
                                    "lines longer than 2700 (2000 with SA) are not supported");
+    if (scan_only && (jmax > 2048 || kmax > 2048))
+        return fail(SMAC_ERR_SIZE, "the scan form of the line solves takes lines of up to 2048 points");
     std::string why;
     if (!check_gpu(why)) return fail(SMAC_ERR_NOGPU, why);
     const long N = (long)jmax * kmax;
@@ -308,6 +317,15 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
     for (double **pp : sa_planes) TRY(dalloc(h, pp, N));
     TRY(dalloc(h, &h->st, 1));
     if (p->impsch == 2) TRY(alloc_lusgs_ws(h, h->lusgs_rows));
+    if (p->impsch == 1) {
+        char *lb = nullptr;
+        TRY(dalloc(h, &lb, (size_t)lr_scan_bytes(h->d)));
+        h->lrs.op = lb;
+        h->lrs.s_t = (double *)(lb + 45 * N * 8 + 256);
+        h->lrs.dq_t = h->lrs.s_t + 3 * N;
+        h->lrs.sa_op = (double *)((char *)(h->lrs.dq_t + 3 * N) + 256);
+        h->lr_solve = scan_only ? 1 : 0;
+    }
     h->w.nred = 512;
     TRY(dalloc(h, &h->w.red_v, 2 * h->w.nred));
     TRY(dalloc(h, &h->w.red_i, 2 * h->w.nred));
@@ -395,6 +413,16 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
         h->lws.ring = value;
         return SMAC_OK;
     }
+    if (!std::strcmp(name, "lr_solve")) {
+        if (value != 0 && value != 1) return fail(SMAC_ERR_ARG, "lr_solve must be 0 or 1");
+        if (h->p.impsch != 1) return fail(SMAC_ERR_ARG, "lr_solve applies to line relaxation (impsch = 1)");
+        if (!value && (h->p.precision == SMAC_FP32_LHS || h->p.line_order == SMAC_ORDER_ZEBRA))
+            return fail(SMAC_ERR_ARG, "FP32-LHS and zebra line relaxation exist only with lr_solve = 1");
+        if (value && (h->d.jm > 2048 || h->d.km > 2048))
+            return fail(SMAC_ERR_SIZE, "the scan form of the line solves takes lines of up to 2048 points");
+        h->lr_solve = value;
+        return SMAC_OK;
+    }
     if (!std::strcmp(name, "lr_lines")) {
         if (value < 1 || value > 64) return fail(SMAC_ERR_ARG, "lr_lines must be in [1, 64]");
         h->lr_lines = value;
@@ -443,6 +471,27 @@ static void mark(smac_handle *h, int i)
     if (h->timing) (void)hipEventRecord(h->ev[i], h->stream);
 }
 
+// the line-relaxation solve of a pass: the scan form (FP32 or FP64 blocks, reference or zebra
+// line order) or the reference's sequential substitution
+static void enqueue_lrelax(smac_handle *h, hipStream_t sm)
+{
+    const Dims &d = h->d;
+    double *btc = h->blk[0], *bjm = h->blk[1], *bjp = h->blk[2], *bkm = h->blk[3],
+           *bkp = h->blk[4];
+    const bool zebra = h->p.line_order == SMAC_ORDER_ZEBRA;
+    if (!h->lr_solve)
+        launch_lrelax_ref<double>(sm, d, h->dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
+                                  h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lr_lines);
+    else if (h->p.precision == SMAC_FP32_LHS)
+        launch_lrelax_scan<float>(sm, d, h->dp, h->p.njsp, h->p.nksp, h->p.nbcimp, zebra, h->wake,
+                                  h->q, h->dq, h->s, (const float *)btc, (const float *)bjm,
+                                  (const float *)bjp, (const float *)bkm, (const float *)bkp,
+                                  h->lrs);
+    else
+        launch_lrelax_scan<double>(sm, d, h->dp, h->p.njsp, h->p.nksp, h->p.nbcimp, zebra,
+                                   h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lrs);
+}
+
 // one pass of the time loop body: SA, assembly, BC rows, residual, solve, update + forces
 static smac_status enqueue_pass(smac_handle *h)
 {
@@ -466,7 +515,8 @@ static smac_status enqueue_pass(smac_handle *h)
         launch_sa_assemble(ss, d, dp, h->first_sa ? 1 : 0, h->q, h->met, h->dj,
                            (int)h->regs.size(), h->regs.data(), h->sa);
         if (side) HIP_TRY(hipEventRecord(h->ev_sa_read, ss));  // done reading q
-        launch_sa_solve(ss, d, dp, h->p.ntjsp, h->wake, &h->lws_sa, h->sa);
+        launch_sa_solve(ss, d, dp, h->p.ntjsp, h->wake, &h->lws_sa, h->sa,
+                        h->lr_solve ? &h->lrs : nullptr);
         launch_sa_update(ss, d, dp, h->w, h->st, h->sa);
         if (side) HIP_TRY(hipEventRecord(h->ev_sa_done, ss));
         h->first_sa = false;
@@ -492,8 +542,7 @@ static smac_status enqueue_pass(smac_handle *h)
             HIP_TRY(hipMemcpyAsync(h->lhs_copy[b], h->blk[b], 9 * d.N * 8, hipMemcpyDeviceToDevice, sm));
     mark(h, 3);
     if (h->p.impsch == 1)
-        launch_lrelax_ref<double>(sm, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
-                                  h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lr_lines);
+        enqueue_lrelax(h, sm);
     else if (f32)
         launch_lusgs3<float>(sm, d, h->lws, h->wake, h->q, h->dq, h->s, (const float *)btc,
                              (const float *)bjm, (const float *)bjp, (const float *)bkm,
@@ -815,7 +864,8 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
         if (is_lusgs(h)) {
             lusgs_pre<double, 1>(ls, d, h->lws_sa, a.sf, a.stc, a.sjm, a.sjp, a.skm, a.skp, nullptr);
         } else {
-            launch_sa_solve(ls, d, h->dp, h->p.ntjsp, h->wake, &h->lws_sa, a);
+            launch_sa_solve(ls, d, h->dp, h->p.ntjsp, h->wake, &h->lws_sa, a,
+                            h->lr_solve ? &h->lrs : nullptr);
             launch_sa_update(ls, d, h->dp, h->w, h->st, a);
         }
     }
@@ -860,8 +910,7 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
         }
         launch_resid(ls, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
         if (!is_lusgs(h))
-            launch_lrelax_ref<double>(ls, d, dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake,
-                                      h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lr_lines);
+            enqueue_lrelax(h, ls);
         else if (f32)
             lusgs_pre<float, 3>(ls, d, h->lws, h->s, fb[0], fb[1], fb[2], fb[3], fb[4], nullptr);
         else

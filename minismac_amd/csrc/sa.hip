@@ -414,9 +414,11 @@ void launch_sa_assemble(hipStream_t sm, const Dims &d, const DevParams &prm, int
 }
 
 void launch_sa_solve(hipStream_t sm, const Dims &d, const DevParams &prm, int ntjsp,
-                     const WakePairs &w, const LusgsWs *ws, SaArrays &a)
+                     const WakePairs &w, const LusgsWs *ws, SaArrays &a, const LrScan *scan)
 {
-    if (prm.impsch == 1) {
+    if (prm.impsch == 1 && scan) {
+        launch_sa_lreltur_scan(sm, d, ntjsp, w, a, *scan);
+    } else if (prm.impsch == 1) {
         hipLaunchKernelGGL(k_sa_lreltur, dim3(1), dim3(SA_THREADS), sizeof(double) * 4 * d.jm, sm,
                            d, ntjsp, w.n, w.pairs, a.anut, a.ds, a.sf, a.sjm, a.sjp, a.skm, a.skp,
                            a.stc);

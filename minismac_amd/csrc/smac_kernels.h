@@ -50,6 +50,20 @@ void launch_lrelax_ref(hipStream_t, const Dims &, const DevParams &, int njsp, i
 void launch_wake(hipStream_t, const Dims &, int neqs, const WakePairs &, const double *sq,
                  double *ds, double *rhs);
 
+// linescan.hip: line relaxation with each line's solve parallel inside the line (prefix scans
+// of the substitutions' affine maps), in the reference's line order or the zebra order
+struct LrScan {
+    void *op;            // 45 line-major operand planes (LT)
+    double *s_t, *dq_t;  // eta-major s and dq (3 planes each)
+    double *sa_op;       // 5 planes of the SA's scalar line system
+};
+long lr_scan_bytes(const Dims &);
+template <typename LT>
+void launch_lrelax_scan(hipStream_t, const Dims &, const DevParams &, int njsp, int nksp,
+                        int nbcimp, bool zebra, const WakePairs &, const double *q, double *dq,
+                        double *s, const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm,
+                        const LT *bkp, const LrScan &);
+
 // lusgs.hip: LU-SGS with hyperplane (j+k = const) sweeps (ref/lusgs.f, ref/ilusol.f).
 // Workspace in the sheared layout: point (j,k) at (j+k+16)*ks + k of every plane, planes
 // stored in pairs for the 3x3 system.
@@ -118,7 +132,9 @@ void launch_sa_assemble(hipStream_t, const Dims &, const DevParams &, int first_
                         const double *q, const double *met, const double *dj, int nreg,
                         const BcRegion *regs, SaArrays &);
 void launch_sa_solve(hipStream_t, const Dims &, const DevParams &, int ntjsp, const WakePairs &,
-                     const LusgsWs *, SaArrays &);
+                     const LusgsWs *, SaArrays &, const LrScan *scan = nullptr);
+void launch_sa_lreltur_scan(hipStream_t, const Dims &, int ntjsp, const WakePairs &, SaArrays &,
+                            const LrScan &);
 void launch_sa_update(hipStream_t, const Dims &, const DevParams &, Scratch &, DevStats *,
                       SaArrays &);
 

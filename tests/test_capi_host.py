@@ -47,6 +47,7 @@ def _create(**kw):
 @pytest.mark.parametrize("kw,code", [
     (dict(impsch=3), 2), (dict(iflxo=5), 3), (dict(underr=0.0), 1), (dict(underr=1.5), 1),
     (dict(ivis=2), 1), (dict(ntksp=1), 1), (dict(dtau=-1.0), 1),
+    (dict(line_order=2), 1), (dict(line_order=1, impsch=2), 1),
 ])
 def test_bad_settings_return_status(kw, code):
     rc, msg = _create(**kw)

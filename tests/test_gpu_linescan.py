@@ -1,0 +1,104 @@
+"""Line relaxation with each line's block-tridiagonal solve done by prefix scans inside the
+line (option lr_solve = 1; the only form of FP32-LHS and zebra line relaxation).
+
+The scan reassociates the block-Thomas substitutions, so the bar is the north_star's FP64
+one: per-pass residual norms (||s||_2, resmax, divmax) and CL/CD within 1e-10 relative of the
+oracle over whole trajectories, q within 1e-10 of its scale; the SA's lreltur (same scan,
+1x1 blocks) is held per pass from the oracle's state as the chain form is (anut to 1e-10).
+FP32-LHS and the zebra order are judged at convergence (a 1e-12 drop of ||s||_2, so that
+what is left of the iteration error stays below the bars): CL/CD equal to the FP64
+reference-order path within 1e-6 (FP32-LHS) and 1e-9 (zebra, a different iteration with the
+same fixed point, and the scan form).  Steady case: 119x31, Re = 1000, dtau = 10,
+under-relaxation 1 (the reference's 0.01 would take a hundred times more passes).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+from minismac_amd._abi import SMAC_FP32_LHS, SMAC_ORDER_ZEBRA
+from minismac_amd.case import config_case, shipped_case
+from minismac_amd.solver import SmacError, Solver
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+SHIPPED_XY = os.path.join(GOLDEN, "shipped_xy_119x31.dat")
+SHIPPED_BC = os.path.join(GOLDEN, "shipped_bcmain_119x31.dat")
+REL = 1e-10
+
+
+def _close(g, o, tag):
+    for k in ("rhsnorm", "resmax", "divmax", "cl", "cd", "cm"):
+        assert g[k] == pytest.approx(o[k], rel=REL, abs=1e-300), f"{tag}: {k} {g[k]!r} {o[k]!r}"
+
+
+@pytest.mark.parametrize("name,npass", [("shipped", 100), ("c1", 100), ("c2", 2), ("c3", 1)])
+def test_scan_line_relaxation_tracks_oracle(name, npass):
+    case = {"shipped": lambda: shipped_case(SHIPPED_XY, SHIPPED_BC, impsch=1),
+            "c1": lambda: config_case(0, impsch=1),
+            "c2": lambda: config_case(1, impsch=1),
+            "c3": lambda: config_case(2, impsch=1)}[name]()
+    print(f"scan line relaxation {case.name}: {npass} passes", flush=True)
+    o = pyoracle.Oracle(case)
+    with Solver(case, options={"lr_solve": 1}) as g:
+        for it in range(npass):
+            _close(g.step(), o.step(), f"pass {it + 1}")
+        q, _, _ = g.get_state()
+    qo = o.array("q")
+    assert np.abs(q - qo).max() <= REL * np.abs(qo).max()
+
+
+@pytest.mark.parametrize("nt", [1, 10, 40])
+def test_scan_sa_lreltur_single_pass_from_oracle_state(nt):
+    case = shipped_case(SHIPPED_XY, SHIPPED_BC, impsch=1)
+    o = pyoracle.Oracle(case)
+    prev = None
+    for _ in range(nt - 1):
+        prev = o.step()
+    q0, a0 = o.array("q"), o.array("anut")
+    ost = o.step()
+    with Solver(case, options={"lr_solve": 1}) as g:
+        if nt > 1:
+            g.set_state(q=q0, anut=a0, nt=nt - 1, resmax0=prev["resmax0"], clift=prev["cl"])
+        gst = g.step()
+        _, anut, _ = g.get_state()
+    _close(gst, ost, f"nt={nt}")
+    assert gst["turres"] == pytest.approx(ost["turres"], rel=1e-10, abs=1e-300)
+    ao = o.array("anut")
+    np.testing.assert_allclose(anut, ao, rtol=1e-10, atol=1e-12 * np.abs(ao).max())
+
+
+def _steady(**kw):
+    return config_case(0, impsch=1, reynum=1000.0, dtau=10.0, underr=1.0, **kw)
+
+
+def _converge(case, options=None, maxit=30000):
+    with Solver(case, options=options) as g:
+        st, n = g.run(maxit, drop=1e-12)
+    assert n < maxit, f"{case.name}: no 1e-12 drop in {maxit} passes"
+    return st, n
+
+
+def test_fp32lhs_and_zebra_line_relaxation_converge_to_the_fp64_state():
+    ref, n_ref = _converge(_steady())
+    f32, n_f32 = _converge(_steady(precision=SMAC_FP32_LHS))
+    zeb, n_zeb = _converge(_steady(line_order=SMAC_ORDER_ZEBRA))
+    scan, n_scan = _converge(_steady(), options={"lr_solve": 1})
+    print(f"passes to a 1e-12 drop: chain {n_ref}, scan {n_scan}, fp32-lhs {n_f32}, "
+          f"zebra {n_zeb}; cl {ref['cl']!r} {scan['cl']!r} {f32['cl']!r} {zeb['cl']!r}",
+          flush=True)
+    for k in ("cl", "cd"):
+        assert scan[k] == pytest.approx(ref[k], rel=1e-9)
+        assert f32[k] == pytest.approx(ref[k], rel=1e-6)
+        assert zeb[k] == pytest.approx(ref[k], rel=1e-9)
+
+
+def test_fp32lhs_and_zebra_need_the_scan_form():
+    for kw in (dict(precision=SMAC_FP32_LHS), dict(line_order=SMAC_ORDER_ZEBRA)):
+        with Solver(_steady(**kw)) as g:
+            with pytest.raises(SmacError):
+                g.set_option("lr_solve", 0)
+            g.step()
