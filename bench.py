@@ -4,8 +4,20 @@ One step = one pass of the reference's time-loop body (SA, residual and implicit
 assembly, BC rows, residual norms, LU-SGS or line relaxation, update, wall forces) on the
 945x241 NACA 4412 C-grid (wall spacing 2e-6 c, Re = 3e6, alpha = 8 deg).  value = whole-job
 Mcell-iterations/s (cells x passes / wall time) of the FP64 path, which performs the
-reference's arithmetic, with the case resident in HBM.  The FP32-LHS mode (FP32 Jacobians,
-factors and LU-SGS sweeps; FP64 residual and update) is timed beside it.  Prints one JSON line.
+reference's arithmetic, with the case resident in HBM.  Beside it:
+  - fp32_lhs: the same per pass with FP32 Jacobians, factors and LU-SGS sweeps (FP64
+    residual and update);
+  - line_relaxation: the line-relaxation pass in its three forms (reference order with the
+    sequential substitution, bit-identical; reference order with the line solves as prefix
+    scans; zebra order), FP64 and FP32-LHS;
+  - converged: the metric itself -- wall-clock to a 1e-10 drop of ||s||_2 and the CL/CD it
+    converges to, FP64 against FP32-LHS, on the steady variant of the grid (Re = 1000: the
+    reference's mean flow is laminar, and at Re = 3e6 it never settles);
+  - adjoint: gradients of CL and CD in alpha and in eight Hicks-Henne bump amplitudes at that
+    converged FP64 state, and the adjoint/primal time;
+  - batched: the 64-case seeded batch;
+  - cpu_baseline: the oracle on one core.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -25,11 +37,17 @@ PEAK_HBM_GBS = 8000.0
 # premultiplied by the inverse diagonal, no factors): 18 + 3 words read and 3 written, all at
 # 4 bytes.
 BAND_FWD_BYTES = {"fp64": 33 * 8, "fp32": 24 * 4}
-# HBM bytes per grid point that the same kernel moves, measured at 945x241 with
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (scaled by 2.0 and 1.0: the factors that make
-# the two counters read 256 MiB for a plain 256 MiB device copy in the same run)
+# HBM bytes per grid point that the same kernel moves: a fixed figure, not measured by this
+# run -- rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE at 945x241 (scaled by 2.0 and 1.0, the
+# factors that make the two counters read 256 MiB for a plain 256 MiB device copy), recorded
+# in profiles/lusgs_band_945x241_counters.md
 BAND_FWD_TRAFFIC = {"fp64": (69081.0 + 8225.0) * 1024 / 227745,
                     "fp32": (28733.0 + 4207.0) * 1024 / 227745}
+TRAFFIC_SOURCE = "fixed figure from profiles/lusgs_band_945x241_counters.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+# Algorithmic bytes per grid point of the line-relaxation solve family (two xi and two eta
+# sweeps with their factorisations): 360 + 2*432 per direction in FP64, 180 + 2*228 with the
+# blocks in FP32
+LR_SOLVE_BYTES = {"fp64": 2448, "fp32": 1272}
 
 
 def cpu_baseline(case, iters: int) -> dict:
@@ -67,6 +85,92 @@ def kernel_times(case, steps: int, warmup: int) -> dict:
         s.run(warmup)
         s.run(steps)
         return s.timings()
+
+
+def line_relaxation_run(cfg: int, steps: int) -> dict:
+    """The line-relaxation pass at the workload size in its three forms."""
+    from minismac_amd._abi import SMAC_FP32_LHS, SMAC_ORDER_ZEBRA
+    from minismac_amd.case import config_case
+    from minismac_amd.solver import Solver
+    out = {}
+    forms = [("chain_fp64", {}, {}, 2), ("scan_fp64", {}, {"lr_solve": 1}, steps),
+             ("scan_fp32", {"precision": SMAC_FP32_LHS}, {}, steps),
+             ("zebra_fp64", {"line_order": SMAC_ORDER_ZEBRA}, {}, steps),
+             ("zebra_fp32", {"line_order": SMAC_ORDER_ZEBRA, "precision": SMAC_FP32_LHS}, {},
+              steps)]
+    for name, kw, opts, n in forms:
+        case = config_case(cfg, impsch=1, **kw)
+        with Solver(case, options=opts, timing=True) as s:
+            s.run(1)
+            t0 = time.perf_counter()
+            st, done = s.run(n)
+            wall = time.perf_counter() - t0
+            tm = s.timings()
+        prec = "fp32" if "fp32" in name else "fp64"
+        out[name] = {"ms_per_pass": round(wall / done * 1e3, 4),
+                     "solve_ms": round(tm["solve"], 4), "sa_ms": round(tm["sa"], 4),
+                     "mcell_iters_s": round(case.npts * done / wall / 1e6, 4),
+                     "solve_achieved_GBs": round(LR_SOLVE_BYTES[prec] * case.npts /
+                                                 (tm["solve"] * 1e-3) / 1e9, 3)}
+    out["note"] = ("chain = the reference's sequential block substitution (bit-identical to "
+                   "the oracle); scan = the same line order with each line solved by prefix "
+                   "scans (per-pass norms within 1e-10 of the oracle); zebra = even then odd "
+                   "lines, each colour at once; one stream with HIP-event timing of the "
+                   "families")
+    return out
+
+
+STEADY = dict(reynum=1000.0, dtau=10.0, underr=1.0)
+
+
+def converged_run(cfg: int, maxit: int, drop: float, adjoint: bool) -> dict:
+    """Wall-clock to a `drop` fall of ||s||_2 from the free stream, FP64 and FP32-LHS, zebra
+    line relaxation, on the steady variant of the workload grid; then the adjoint gradients
+    at the FP64 state."""
+    from minismac_amd import grid
+    from minismac_amd._abi import SMAC_FP32_LHS, SMAC_ORDER_ZEBRA
+    from minismac_amd.case import config_case
+    from minismac_amd.solver import Solver
+    out = {}
+    res = {}
+    for prec, name in ((0, "fp64"), (SMAC_FP32_LHS, "fp32_lhs")):
+        case = config_case(cfg, impsch=1, line_order=SMAC_ORDER_ZEBRA, precision=prec, **STEADY)
+        s = Solver(case)
+        try:
+            s.step()                              # the first pass (wall distance)
+            t0 = time.perf_counter()
+            st, done = s.run(maxit, drop=drop)
+            wall = time.perf_counter() - t0
+            res[name] = {"passes": done + 1, "seconds": round(wall, 3),
+                         "mcell_iters_s": round(case.npts * done / wall / 1e6, 4),
+                         "converged": bool(done < maxit), "cl": st["cl"], "cd": st["cd"],
+                         "rhsnorm": st["rhsnorm"]}
+            if name == "fp64" and adjoint:
+                jl, ju = [(r[1], r[2]) for r in case.resolved_regions() if r[0] == 0][0]
+                modes = grid.bump_modes(case.x, case.y, jl, ju)
+                dcl, dcd, ast = s.adjoint(modes, maxit=4000, tol=1e-10)
+                out["adjoint"] = {
+                    "seconds": round(ast["seconds"], 3), "gmres_iters": ast["iters"],
+                    "resid": ast["resid"],
+                    "adjoint_over_primal": round(ast["seconds"] / wall, 4),
+                    "dcl_dalpha_per_deg": dcl[0], "dcd_dalpha_per_deg": dcd[0],
+                    "dcl_dbumps": list(dcl[1:]), "dcd_dbumps": list(dcd[1:]),
+                    "note": "both adjoints (CL, CD) by GMRES(40) with the transposed LU-SGS "
+                            "update as right preconditioner, plus the gradients of 8 "
+                            "Hicks-Henne bump modes (peaks 0.1, 0.3, 0.5, 0.7 on each side, "
+                            "algebraic mesh deformation); primal = FP64 zebra passes from the "
+                            "free stream to the same state"}
+        finally:
+            s.close()
+    out.update(res)
+    a, b = res["fp64"], res["fp32_lhs"]
+    out["speedup_fp32_lhs"] = round(a["seconds"] / b["seconds"], 4)
+    out["cl_rel_diff"] = abs(b["cl"] - a["cl"]) / abs(a["cl"])
+    out["cd_rel_diff"] = abs(b["cd"] - a["cd"]) / abs(a["cd"])
+    out["workload"] = (f"945x241 NACA 4412 C-grid, Re=1000 (steady), alpha=8, beta=10, SA, "
+                       f"dtau=10, under-relaxation 1, zebra line relaxation, to a {drop:g} drop "
+                       f"of ||s||_2 from the free stream")
+    return out
 
 
 def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
@@ -129,6 +233,11 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-LHS measurement")
     ap.add_argument("--batch-cases", type=int, default=64,
                     help="cases in the batched measurement (0 skips it)")
+    ap.add_argument("--no-lr", action="store_true", help="skip the line-relaxation forms")
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the converged FP64 / FP32-LHS comparison and the adjoint")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradients")
+    ap.add_argument("--converge-maxit", type=int, default=12000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -187,6 +296,7 @@ def main() -> int:
             "achieved": round(achieved, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 6),
             "traffic": round(BAND_FWD_TRAFFIC["fp64"] * case.npts),
+            "traffic_source": TRAFFIC_SOURCE,
             "alg_bytes_per_launch": alg, "avg_ms_per_launch": round(t, 5),
         }
     else:
@@ -209,9 +319,16 @@ def main() -> int:
             "band_fwd_ms": round(tm32["band_fwd"], 5),
             "band_fwd_achieved_GBs": round(BAND_FWD_BYTES["fp32"] * c32.npts / (tm32["band_fwd"] * 1e-3) / 1e9, 3),
             "band_fwd_traffic": round(BAND_FWD_TRAFFIC["fp32"] * c32.npts),
-            "note": "converged CL/CD equal to the FP64 path within 1e-6 on a steady case "
-                    "(this Re=3e6 flow is unsteady: the reference's mean flow is laminar)",
+            "note": "per pass on the Re=3e6 flow, which never settles (the reference's mean "
+                    "flow is laminar); time to the same converged CL/CD is under 'converged'",
         }
+    if not args.no_lr:
+        print("bench: line relaxation forms", file=sys.stderr, flush=True)
+        line["line_relaxation"] = line_relaxation_run(args.config, args.steps // 4 or 1)
+    if not args.no_converge:
+        print("bench: converged FP64 / FP32-LHS and adjoint", file=sys.stderr, flush=True)
+        line["converged"] = converged_run(args.config, args.converge_maxit, 1e-10,
+                                          not args.no_adjoint)
     if args.batch_cases > 0:
         print(f"bench: batched workload, {args.batch_cases} cases", file=sys.stderr, flush=True)
         line["batched"] = batched_run(args.batch_cases, args.steps, args.warmup)

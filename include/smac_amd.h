@@ -159,6 +159,36 @@ smac_status smac_timings(smac_handle *h, double out[7]);
  * No counterpart in the reference. */
 smac_status smac_set_option(smac_handle *h, const char *name, int value);
 
+/* ---- Discrete adjoint (no counterpart in the reference, which computes no sensitivities) --
+ * Gradients of CL and CD at the case's current state, which must be a steady state of the
+ * iteration (run it to a tight drop first).  The steady equations are the assembled residual
+ * s = 0 on every row but the wake-cut targets, the wake-cut relation
+ * q(jt,1) = (q(jt,2) + q(jb,2)) / 2 there (what the injection ref/bc.f:571-591 enforces), and
+ * clift = CL (the far field reads the CL of the pass before).  The adjoint of that system is
+ * solved once for CL and once for CD with GMRES, right-preconditioned by the transpose of the
+ * iteration's own LU-SGS update; the mean flow does not read the SA variables, so the SA part
+ * of the adjoint vanishes.
+ *  - dcl[0], dcd[0]: d/d(alpha), alpha in degrees;
+ *  - dcl[1+i], dcd[1+i]: along shape direction i, given as grid displacements dx, dy
+ *    (nshape x jmax*kmax each, Fortran order; the metrics follow them as the reference forms
+ *    them); nshape may be 0.
+ * maxit bounds the GMRES iterations of each solve, tol is the relative residual to reach. */
+typedef struct smac_adjoint_stats {
+    int iters[2];      /* GMRES iterations of the CL and the CD adjoint                        */
+    double resid[2];   /* their final residuals relative to the right-hand side             */
+    double rhsnorm;    /* ||s||_2 of the state the gradients are taken at                   */
+    double seconds;    /* wall time of the call                                              */
+} smac_adjoint_stats;
+smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const double *dy,
+                         int maxit, double tol, double *dcl, double *dcd,
+                         smac_adjoint_stats *st);
+/* The operators of the last smac_adjoint call applied to a host vector (3 planes):
+ * op 0: F_q^T in; 1: the adjoint operator -F_q^T in - (in . F_c) CL_q; 2: P^T in, the
+ * transposed update; 3: P in, the iteration's own update (LU-SGS solve and wake injection).
+ * For checks of the adjoint (dot-product tests); arrays "adj_jac" (81 planes: 9 offsets x 9
+ * entries), "adj_lam_cl", "adj_lam_cd", "adj_gcl", "adj_gcd" come from smac_get_array. */
+smac_status smac_adjoint_apply(smac_handle *h, int op, const double *in, double *out);
+
 /* ---- Tier 1: drop-ins at the reference's own Fortran call sites (host arrays) --------- */
 
 /* lrelax(jmax,kmax,q,dq,s,btc,bjm,bjp,bkm,bkp), ref/lrelax.f:51-221, called at

@@ -63,3 +63,12 @@ def make_params(**kw) -> SmacParams:
 
 def params_dict(p: SmacParams) -> dict:
     return {name: getattr(p, name) for name, _ in p._fields_}
+
+
+class SmacAdjointStats(C.Structure):
+    _fields_ = [("iters", C.c_int * 2), ("resid", C.c_double * 2), ("rhsnorm", C.c_double),
+                ("seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"iters": list(self.iters), "resid": list(self.resid), "rhsnorm": self.rhsnorm,
+                "seconds": self.seconds}

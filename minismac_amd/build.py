@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libsmac_amd.so")
 SOURCES = ["api.hip", "assemble.hip", "linerelax.hip", "linescan.hip", "lusgs.hip", "sa.hip",
-           "forces.hip", "noalloc.hip"]
+           "forces.hip", "noalloc.hip", "adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: the kernels keep the reference's operation order (no FMA contraction),

@@ -3,6 +3,7 @@
 // the reference's lrelax / lusgs call sites.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -62,6 +63,11 @@ struct smac_handle {
     // reference's sequential substitution (0, bit-identical to the reference)
     int lr_solve = 0;
     LrScan lrs = {};
+    // discrete adjoint (allocated by the first smac_adjoint call)
+    bool adj_ready = false;
+    int adj_m = 0;
+    AdjCtx adj = {};
+    LusgsWs adj_ws = {};
     std::vector<void *> lws_allocs;
     Walls walls = {};
     std::vector<BcRegion> regs;
@@ -114,6 +120,7 @@ static void free_lusgs_ws(smac_handle *h)
     h->lws_allocs.clear();
     h->lws = LusgsWs{};
     h->lws_sa = LusgsWs{};
+    h->adj_ws = LusgsWs{};
 }
 
 static smac_status alloc_one_ws(smac_handle *h, int rows, int neqs, LusgsWs &w)
@@ -668,14 +675,180 @@ smac_status smac_get_array(smac_handle *h, const char *name, double *out)
         {"btc", h->lhs_copy[0], 9 * N}, {"bjm", h->lhs_copy[1], 9 * N},
         {"bjp", h->lhs_copy[2], 9 * N}, {"bkm", h->lhs_copy[3], 9 * N},
         {"bkp", h->lhs_copy[4], 9 * N},
+        {"adj_jac", h->adj_ready ? h->adj.jac : nullptr, 81 * N},
+        {"adj_lam_cl", h->adj_ready ? h->adj.lam[0] : nullptr, 3 * N},
+        {"adj_lam_cd", h->adj_ready ? h->adj.lam[1] : nullptr, 3 * N},
+        {"adj_gcl", h->adj_ready ? h->adj.g[0] : nullptr, 3 * N},
+        {"adj_gcd", h->adj_ready ? h->adj.g[1] : nullptr, 3 * N},
     };
     for (auto &e : t)
         if (!std::strcmp(e.n, name)) {
-            if (!e.p) return fail(SMAC_ERR_ARG, std::string(name) + " is kept only with option keep_lhs");
+            if (!e.p)
+                return fail(SMAC_ERR_ARG, std::string(name) + (name[0] == 'a'
+                                                   ? " exists after a smac_adjoint call"
+                                                   : " is kept only with option keep_lhs"));
             HIP_TRY(hipMemcpy(out, e.p, e.cnt * 8, hipMemcpyDeviceToHost));
             return SMAC_OK;
         }
     return fail(SMAC_ERR_ARG, std::string("unknown array ") + name);
+}
+
+}  // extern "C"
+
+// ---- discrete adjoint ------------------------------------------------------------------
+
+// row types of the steady equations (the BC rows in file order, then the wake-cut targets)
+static smac_status adj_row_types(smac_handle *h, std::vector<signed char> &rt,
+                                 std::vector<int> &inv)
+{
+    const int jm = h->d.jm, km = h->d.km;
+    rt.assign(h->d.N, (signed char)-1);
+    for (int k = 1; k < km - 1; ++k)
+        for (int j = 1; j < jm - 1; ++j) rt[(long)j + (long)jm * k] = 0;
+    for (const BcRegion &r : h->regs)
+        for (int k = r.kbeg; k <= r.kend; ++k)
+            for (int j = r.jbeg; j <= r.jend; ++j) {
+                const long p = (long)j + (long)jm * k;
+                if (r.ibcval == 31) rt[p] = j == 0 ? 1 : 2;
+                else if (r.ibcval == 0) rt[p] = 3;
+                else if (r.ibcval == 25) rt[p] = 4;
+            }
+    const int np = h->wake.n;
+    std::vector<int> pr(2 * np);
+    if (np > 0 && hipMemcpy(pr.data(), h->pairs, 2 * np * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(SMAC_ERR_HIP, "reading the wake pairs failed");
+    inv.assign(np, -1);
+    for (int i = 0; i < np; ++i) {
+        const long p = pr[2 * i];
+        if (rt[p] != -1)
+            return fail(SMAC_ERR_BC, "a wake-cut point also carries a BC row: not a steady system the adjoint covers");
+        rt[p] = 5;
+        for (int i2 = 0; i2 < np; ++i2)
+            if (pr[2 * i2 + 1] == pr[2 * i]) inv[i] = i2;
+        if (inv[i] < 0) return fail(SMAC_ERR_BC, "wake cut: a target that is no pair's partner");
+    }
+    for (long p = 0; p < h->d.N; ++p)
+        if (rt[p] < 0) return fail(SMAC_ERR_BC, "a boundary point without a BC row or wake-cut relation");
+    return SMAC_OK;
+}
+
+static smac_status adj_prepare(smac_handle *h, int m)
+{
+    const long N = h->d.N;
+    if (!h->adj_ready || h->adj_m != m) {
+        AdjCtx &c = h->adj;
+        c = AdjCtx{};
+        std::vector<signed char> rt;
+        std::vector<int> inv;
+        smac_status rc = adj_row_types(h, rt, inv);
+        if (rc) return rc;
+        signed char *rtd = nullptr;
+        int *invd = nullptr;
+        if ((rc = dalloc(h, &rtd, N)) || (rc = dalloc(h, &invd, inv.size() + 1))) return rc;
+        HIP_TRY(hipMemcpy(rtd, rt.data(), N, hipMemcpyHostToDevice));
+        if (!inv.empty()) HIP_TRY(hipMemcpy(invd, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice));
+        c.rtype = rtd;
+        c.inv = invd;
+        double **v3[] = {&c.lam[0], &c.lam[1], &c.g[0], &c.g[1], &c.fc, &c.fa, &c.w, &c.z,
+                         &c.t1, &c.t2, &c.t1b, &c.fb};
+        if ((rc = dalloc(h, &c.jac, 81 * N))) return rc;
+        for (int b = 0; b < 5; ++b)
+            if ((rc = dalloc(h, &c.tb[b], 9 * N))) return rc;
+        for (double **pp : v3)
+            if ((rc = dalloc(h, pp, 3 * N))) return rc;
+        if ((rc = dalloc(h, &c.V, 3 * N * (m + 1)))) return rc;
+        if ((rc = dalloc(h, &c.dmet, 4 * N)) || (rc = dalloc(h, &c.ddj, N)) ||
+            (rc = dalloc(h, &c.dxy, 2 * N)) || (rc = dalloc(h, &c.fpart, 2 * h->d.jm)) ||
+            (rc = dalloc(h, &c.part, 512 * (m + 2))) || (rc = dalloc(h, &c.red, m + 2)) ||
+            (rc = dalloc(h, &c.red2, m + 2)))
+            return rc;
+        if (!h->adj_ws.lhs) {
+            h->adj_ws.ring = 1;
+            rc = alloc_one_ws(h, 64, 3, h->adj_ws);
+            if (rc) return rc;
+        }
+        h->adj_m = m;
+        h->adj_ready = true;
+    }
+    AdjCtx &c = h->adj;
+    c.d = h->d;
+    c.sm = h->stream;
+    c.x = h->x; c.y = h->y; c.met = h->met; c.dj = h->dj; c.q = h->q;
+    for (int b = 0; b < 5; ++b) c.blk[b] = h->blk[b];
+    c.ws = h->adj_ws;
+    c.walls = h->walls;
+    c.beta = h->dp.beta; c.cdis = h->dp.cdis; c.vnu = h->dp.vnu; c.pout = h->dp.pout;
+    c.alpha = h->dp.alpha; c.chord = h->dp.chord; c.ivis = h->dp.ivis;
+    c.m = m;
+    c.npair = h->wake.n;
+    c.pairs = h->wake.pairs;
+    return SMAC_OK;
+}
+
+extern "C" {
+
+smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const double *dy,
+                         int maxit, double tol, double *dcl, double *dcd, smac_adjoint_stats *st)
+{
+    if (!h || !dcl || !dcd) return fail(SMAC_ERR_ARG, "NULL argument");
+    if (nshape < 0 || (nshape > 0 && (!dx || !dy))) return fail(SMAC_ERR_ARG, "shape directions missing");
+    if (maxit < 1 || !(tol > 0.0)) return fail(SMAC_ERR_ARG, "maxit must be >= 1 and tol > 0");
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream_sa));
+    const int m = 40;
+    smac_status rc = adj_prepare(h, m);
+    if (rc) return rc;
+    const Dims &d = h->d;
+    // the FP64 residual and blocks of the state (the iteration's own operator M)
+    launch_assemble<double>(h->stream, d, h->dp, h->q, h->met, h->dj, h->s, h->blk[0], h->blk[1],
+                            h->blk[2], h->blk[3], h->blk[4]);
+    launch_bc<double>(h->stream, d, h->dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y,
+                      h->met, h->q, h->s, h->blk[0], h->blk[1], h->blk[2], h->blk[3], h->blk[4]);
+    launch_resid(h->stream, d, h->nt > 1 ? h->nt : 2, h->dp.beta, h->s, h->dj, h->w, h->st);   // no resmax0 latch
+    DevStats ds;
+    HIP_TRY(hipMemcpyAsync(&ds, h->st, sizeof ds, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    AdjCtx &c = h->adj;
+    c.clift = ds.clift;
+    c.cl = ds.cl;
+    c.cd = ds.cd;
+    adj_build(c);
+    double rel[2] = {0.0, 0.0};
+    int its[2] = {0, 0};
+    for (int o = 0; o < 2; ++o) its[o] = adj_gmres(c, c.g[o], c.lam[o], maxit, tol, &rel[o]);
+    adj_gradients(c, nshape, dx, dy, dcl, dcd);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (st) {
+        st->iters[0] = its[0];
+        st->iters[1] = its[1];
+        st->resid[0] = rel[0];
+        st->resid[1] = rel[1];
+        st->rhsnorm = ds.rhsnorm;
+        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (!(std::isfinite(dcl[0]) && std::isfinite(dcd[0])))
+        return fail(SMAC_ERR_NONFINITE, "adjoint gradient is not finite");
+    return SMAC_OK;
+}
+
+smac_status smac_adjoint_apply(smac_handle *h, int op, const double *in, double *out)
+{
+    if (!h || !in || !out) return fail(SMAC_ERR_ARG, "NULL argument");
+    if (!h->adj_ready) return fail(SMAC_ERR_ARG, "no adjoint context: call smac_adjoint first");
+    if (op < 0 || op > 3) return fail(SMAC_ERR_ARG, "op must be 0..3");
+    AdjCtx &c = h->adj;
+    const long n3 = 3 * h->d.N;
+    HIP_TRY(hipMemcpy(c.w, in, n3 * 8, hipMemcpyHostToDevice));
+    if (op == 0) adj_apply_jt(c, c.w, c.z);
+    else if (op == 1) adj_apply_op(c, c.w, c.z);
+    else if (op == 2) adj_apply_prec_t(c, c.w, c.z);
+    else adj_apply_prec(c, c.w, c.z);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c.sm));
+    HIP_TRY(hipMemcpy(out, c.z, n3 * 8, hipMemcpyDeviceToHost));
+    return SMAC_OK;
 }
 
 }  // extern "C"

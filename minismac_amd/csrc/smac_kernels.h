@@ -143,4 +143,32 @@ void launch_update_force(hipStream_t, const Dims &, const DevParams &, const Wal
                          const double *x, const double *y, const double *met, const double *dj,
                          double *q, const double *dq, Scratch &, DevStats *);
 
+// adjoint.hip: discrete adjoint of the converged mean flow (CL, CD gradients)
+struct AdjCtx {
+    Dims d;
+    hipStream_t sm;
+    const double *x, *y, *met, *dj, *q;
+    double *blk[5];              // FP64 blocks of the state (the primal's LU-SGS operator M)
+    LusgsWs ws;                  // LU-SGS workspace for M^-T and M^-1
+    Walls walls;
+    double beta, cdis, vnu, pout, alpha, chord, clift, cl, cd;
+    int ivis;
+    int m;                       // GMRES restart length
+    const signed char *rtype;    // row type per point
+    int npair;
+    const int *pairs, *inv;      // wake pairs (0-based) and, per pair, the pair whose partner
+                                 // is its target
+    double *jac, *tb[5], *lam[2], *g[2], *fc, *fa, *V, *w, *z, *t1, *t2, *t1b;
+    double *dmet, *ddj, *dxy, *fb, *fpart;
+    double *part, *red, *red2;
+};
+void adj_build(AdjCtx &);
+void adj_apply_op(AdjCtx &, const double *in, double *out);
+void adj_apply_prec_t(AdjCtx &, const double *in, double *out);
+void adj_apply_prec(AdjCtx &, const double *in, double *out);
+void adj_apply_jt(AdjCtx &, const double *in, double *out);
+int adj_gmres(AdjCtx &, const double *b, double *x, int maxit, double tol, double *rel);
+void adj_gradients(AdjCtx &, int nshape, const double *dx, const double *dy, double *dcl,
+                   double *dcd);
+
 }  // namespace smac

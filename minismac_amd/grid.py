@@ -26,7 +26,8 @@ import numpy as np
 
 __all__ = [
     "CGrid", "naca4_surface", "make_cgrid", "cgrid_bc_regions", "read_plot3d", "write_plot3d",
-    "read_bcmain", "write_bcmain", "write_namelist", "metric", "hicks_henne",
+    "read_bcmain", "write_bcmain", "write_namelist", "metric", "hicks_henne", "bump_modes",
+    "deform",
 ]
 
 
@@ -318,3 +319,51 @@ def metric(x: np.ndarray, y: np.ndarray):
     rtxy[:, :, 1, 1] = -yj
     rtxy[:, :, 1, 2] = xj
     return rtxy, dj
+
+
+BUMP_PEAKS = (0.1, 0.3, 0.5, 0.7)
+
+
+def bump_modes(x: np.ndarray, y: np.ndarray, jte_lower: int, jte_upper: int,
+               peaks=BUMP_PEAKS, width: float = 3.0):
+    """Shape modes of a C-grid: Hicks-Henne bumps on the upper surface (one per peak), then on
+    the lower surface, each carried into the volume by algebraic mesh deformation.
+
+    A bump moves the surface points of its side normal to the chord by
+    ``sin(pi x^(ln 0.5/ln h))^width`` (upward on the upper side, downward on the lower side:
+    positive amplitudes thicken the section, as in ``make_cgrid``'s bumps); a grid point on
+    the j-line of a surface point moves by the same amount times ``(1 - s/S)^3``, s being its
+    distance along the line from the surface and S that of the far-field point, so the far
+    field, the wake cut and the trailing and leading edges stay fixed.  The deformation is
+    linear in the amplitudes: the grid of amplitudes ``a`` is ``(x, y + sum_i a_i dy_i)``.
+    Returns a list of ``(dx, dy)`` pairs, ``2 * len(peaks)`` of them.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    jmax, kmax = x.shape
+    jl, ju = jte_lower - 1, jte_upper - 1          # 0-based trailing-edge points
+    jle = (jl + ju) // 2                            # leading edge (middle of the wall)
+    s = np.sqrt((x - x[:, :1]) ** 2 + (y - y[:, :1]) ** 2)
+    w = (1.0 - s / s[:, -1:]) ** 3
+    modes = []
+    for side in (+1, -1):
+        js = np.arange(jle, ju + 1) if side > 0 else np.arange(jl, jle + 1)
+        for h in peaks:
+            e = math.log(0.5) / math.log(h)
+            xs = np.clip(x[js, 0], 0.0, 1.0)
+            dyw = np.zeros(jmax)
+            dyw[js] = side * np.sin(math.pi * xs ** e) ** width
+            dy = np.asfortranarray(dyw[:, None] * w)
+            modes.append((np.zeros_like(x, order="F"), dy))
+    return modes
+
+
+def deform(x: np.ndarray, y: np.ndarray, modes, amps):
+    """The grid of shape amplitudes ``amps`` (one per mode of ``bump_modes``)."""
+    xn = np.array(x, dtype=np.float64, order="F")
+    yn = np.array(y, dtype=np.float64, order="F")
+    for (dx, dy), a in zip(modes, amps):
+        if a:
+            xn += a * dx
+            yn += a * dy
+    return xn, yn

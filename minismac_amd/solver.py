@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 
-from ._abi import STATUS_NAMES, SmacIterStats, SmacParams, SmacBcRegion
+from ._abi import STATUS_NAMES, SmacAdjointStats, SmacIterStats, SmacParams, SmacBcRegion
 from .case import Case, dptr, fortran
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsmac_amd.so")
@@ -23,7 +23,7 @@ EXPORTS = [
     "smac_create", "smac_destroy", "smac_set_state", "smac_get_state", "smac_step", "smac_run",
     "smac_get_array", "smac_timings", "smac_set_option", "smac_lrelax", "smac_lusgs",
     "smac_get_stats", "smac_batch_create", "smac_batch_run", "smac_batch_timings",
-    "smac_batch_destroy", "smac_lusgs_noalloc",
+    "smac_batch_destroy", "smac_lusgs_noalloc", "smac_adjoint", "smac_adjoint_apply",
     "smac_last_error", "smac_version",
 ]
 
@@ -63,6 +63,9 @@ def lib():
         L.smac_batch_run.argtypes = [C.c_void_p, C.c_int]
         L.smac_batch_timings.argtypes = [C.c_void_p, D]
         L.smac_batch_destroy.argtypes = [C.c_void_p]
+        L.smac_adjoint.argtypes = [C.c_void_p, C.c_int, D, D, C.c_int, C.c_double, D, D,
+                                   P(SmacAdjointStats)]
+        L.smac_adjoint_apply.argtypes = [C.c_void_p, C.c_int, D, D]
         L.smac_lusgs_noalloc.argtypes = [P(SmacParams), C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.c_void_p] + [C.c_void_p] * 11
         for fn in EXPORTS[:-2]:
@@ -79,7 +82,8 @@ def _check(rc: int, where: str) -> None:
 
 
 _PLANES = {"q": 3, "dq": 3, "s": 3, "anut": 1, "vnut": 1, "smin": 1, "sf": 1, "stc": 1,
-           "ds": 1, "vort": 1, "btc": 9, "bjm": 9, "bjp": 9, "bkm": 9, "bkp": 9}
+           "ds": 1, "vort": 1, "btc": 9, "bjm": 9, "bjp": 9, "bkm": 9, "bkp": 9,
+           "adj_jac": 81, "adj_lam_cl": 3, "adj_lam_cd": 3, "adj_gcl": 3, "adj_gcd": 3}
 
 
 class Solver:
@@ -151,6 +155,33 @@ class Solver:
         out = np.zeros((self.jmax, self.kmax, planes) if planes > 1 else (self.jmax, self.kmax),
                        order="F")
         _check(lib().smac_get_array(self.h, name.encode(), dptr(out)), "smac_get_array")
+        return out
+
+    def adjoint(self, shape_dirs=(), maxit: int = 2000, tol: float = 1e-12):
+        """Gradients of CL and CD at the current (steady) state (smac_adjoint): returns
+        (dcl, dcd, stats), dcl[0]/dcd[0] per degree of alpha, then one entry per shape
+        direction, each a (dx, dy) pair of (jmax, kmax) grid displacements."""
+        n = len(shape_dirs)
+        npts = self.jmax * self.kmax
+        dx = np.zeros((max(n, 1), npts))
+        dy = np.zeros((max(n, 1), npts))
+        for i, (ax, ay) in enumerate(shape_dirs):
+            dx[i] = np.asarray(ax, dtype=np.float64).ravel(order="F")
+            dy[i] = np.asarray(ay, dtype=np.float64).ravel(order="F")
+        dcl = np.zeros(1 + n)
+        dcd = np.zeros(1 + n)
+        st = SmacAdjointStats()
+        _check(lib().smac_adjoint(self.h, n, dptr(dx.ravel()), dptr(dy.ravel()), maxit, tol,
+                                  dptr(dcl), dptr(dcd), C.byref(st)), "smac_adjoint")
+        return dcl, dcd, st.as_dict()
+
+    def adjoint_apply(self, op: int, v: np.ndarray) -> np.ndarray:
+        """An operator of the last adjoint call on a (jmax, kmax, 3) field: 0 F_q^T, 1 the
+        adjoint operator, 2 P^T, 3 P (smac_adjoint_apply)."""
+        vin = np.asfortranarray(np.asarray(v, dtype=np.float64).reshape(self.jmax, self.kmax, 3,
+                                                                          order="F"))
+        out = np.zeros_like(vin, order="F")
+        _check(lib().smac_adjoint_apply(self.h, op, dptr(vin), dptr(out)), "smac_adjoint_apply")
         return out
 
     def set_state(self, q=None, anut=None, nt=0, resmax0=1.0, clift=0.0) -> None:

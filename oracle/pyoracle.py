@@ -18,9 +18,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 LIB_PATH = os.path.join(REF_DIR, "libsmac_oracle.so")
 SLIPS_PATH = os.path.join(REF_DIR, "libsmac_oracle_slips.so")
+FP64_PATH = os.path.join(REF_DIR, "libsmac_oracle_fp64.so")
 
 _lib = None
 _slips = None
+_fp64 = None
 
 
 def lib():
@@ -39,6 +41,16 @@ def slips_lib():
         _slips.orc_set_slip.argtypes = [C.c_int]
         _slips.orc_set_slip.restype = None
     return _slips
+
+
+def fp64_lib():
+    """The oracle with the viscous terms' FP32 temporaries kept in FP64: a smooth residual
+    for the FP64 tangent checks of the adjoint (its central differences carry no FP32
+    rounding)."""
+    global _fp64
+    if _fp64 is None:
+        _fp64 = _load(FP64_PATH)
+    return _fp64
 
 
 def _load(path):

@@ -373,6 +373,16 @@ static void conup(orc_case *c)
 /* ------------------------------------------------------------------------------------ */
 /* viscterms: ref/visc.f:50-160 (udif, vdif, vf are default REAL)                         */
 
+/* The default-REAL temporaries of viscterms.  A build with -DORC_FP64_ISLANDS
+ * (_ref/libsmac_oracle_fp64.so) keeps them in double: a smooth residual whose central
+ * differences carry no FP32 rounding, the reference for the FP64 tangent checks of the
+ * adjoint.  The oracle itself is the reference's FP32. */
+#ifdef ORC_FP64_ISLANDS
+typedef double visl_t;
+#else
+typedef float visl_t;
+#endif
+
 static void viscterms(orc_case *c)
 {
     const int jm = c->jm, km = c->km;
@@ -380,15 +390,15 @@ static void viscterms(orc_case *c)
     const double vnu = c->vnu;
     double *q = c->q, *rtxy = c->rtxy, *dj = c->dj, *s = c->s;
     int nmax = jm > km ? jm : km;
-    float *udif = malloc(sizeof(float) * nmax * 3), *vdif = udif + nmax, *vf = vdif + nmax;
+    visl_t *udif = malloc(sizeof(visl_t) * nmax * 3), *vdif = udif + nmax, *vf = vdif + nmax;
     for (int k = 2; k <= km - 1; ++k) {
         for (int j = 1; j <= jm - 1; ++j) {
             double djv = 0.5 * (S2(dj, j, k) * vnu + S2(dj, j + 1, k) * vnu);
-            udif[j - 1] = (float)(Q3(q, j + 1, k, 2) - Q3(q, j, k, 2));
-            vdif[j - 1] = (float)(Q3(q, j + 1, k, 3) - Q3(q, j, k, 3));
+            udif[j - 1] = (visl_t)(Q3(q, j + 1, k, 2) - Q3(q, j, k, 2));
+            vdif[j - 1] = (visl_t)(Q3(q, j + 1, k, 3) - Q3(q, j, k, 3));
             double rx = 0.5 * (RT(j, k, 1, 2) + RT(j + 1, k, 1, 2));
             double ry = 0.5 * (RT(j, k, 1, 3) + RT(j + 1, k, 1, 3));
-            vf[j - 1] = SLIP(SLIP_VISC_RY) ? (float)(djv * (rx * rx)) : (float)(djv * (rx * rx + ry * ry));
+            vf[j - 1] = SLIP(SLIP_VISC_RY) ? (visl_t)(djv * (rx * rx)) : (visl_t)(djv * (rx * rx + ry * ry));
         }
         for (int j = 2; j <= jm - 1; ++j) {
             double a = vf[j - 2], b = vf[j - 1];
@@ -398,9 +408,9 @@ static void viscterms(orc_case *c)
             B9(c->btc, j, k, 3, 3) = B9(c->btc, j, k, 3, 3) + a + b;
             B9(c->bjp, j, k, 2, 2) = B9(c->bjp, j, k, 2, 2) - b;
             B9(c->bjp, j, k, 3, 3) = B9(c->bjp, j, k, 3, 3) - b;
-            float t2 = SLIP(SLIP_VISC_UV) ? vf[j - 1] * vdif[j - 1] - vf[j - 2] * vdif[j - 2]
-                                          : vf[j - 1] * udif[j - 1] - vf[j - 2] * udif[j - 2];
-            float t3 = vf[j - 1] * vdif[j - 1] - vf[j - 2] * vdif[j - 2];
+            visl_t t2 = SLIP(SLIP_VISC_UV) ? vf[j - 1] * vdif[j - 1] - vf[j - 2] * vdif[j - 2]
+                                           : vf[j - 1] * udif[j - 1] - vf[j - 2] * udif[j - 2];
+            visl_t t3 = vf[j - 1] * vdif[j - 1] - vf[j - 2] * vdif[j - 2];
             Q3(s, j, k, 2) = Q3(s, j, k, 2) + (double)t2;
             Q3(s, j, k, 3) = Q3(s, j, k, 3) + (double)t3;
         }
@@ -408,11 +418,11 @@ static void viscterms(orc_case *c)
     for (int j = 2; j <= jm - 1; ++j) {
         for (int k = 1; k <= km - 1; ++k) {
             double djv = 0.5 * (S2(dj, j, k) * vnu + S2(dj, j, k + 1) * vnu);
-            udif[k - 1] = (float)(Q3(q, j, k + 1, 2) - Q3(q, j, k, 2));
-            vdif[k - 1] = (float)(Q3(q, j, k + 1, 3) - Q3(q, j, k, 3));
+            udif[k - 1] = (visl_t)(Q3(q, j, k + 1, 2) - Q3(q, j, k, 2));
+            vdif[k - 1] = (visl_t)(Q3(q, j, k + 1, 3) - Q3(q, j, k, 3));
             double rx = 0.5 * (RT(j, k, 2, 2) + RT(j, k + 1, 2, 2));
             double ry = 0.5 * (RT(j, k, 2, 3) + RT(j, k + 1, 2, 3));
-            vf[k - 1] = (float)(djv * (rx * rx + ry * ry));
+            vf[k - 1] = (visl_t)(djv * (rx * rx + ry * ry));
         }
         for (int k = 2; k <= km - 1; ++k) {
             double a = vf[k - 2], b = vf[k - 1];
@@ -422,8 +432,8 @@ static void viscterms(orc_case *c)
             B9(c->btc, j, k, 3, 3) = B9(c->btc, j, k, 3, 3) + a + b;
             B9(c->bkp, j, k, 2, 2) = B9(c->bkp, j, k, 2, 2) - b;
             B9(c->bkp, j, k, 3, 3) = B9(c->bkp, j, k, 3, 3) - b;
-            float t2 = vf[k - 1] * udif[k - 1] - vf[k - 2] * udif[k - 2];
-            float t3 = vf[k - 1] * vdif[k - 1] - vf[k - 2] * vdif[k - 2];
+            visl_t t2 = vf[k - 1] * udif[k - 1] - vf[k - 2] * udif[k - 2];
+            visl_t t3 = vf[k - 1] * vdif[k - 1] - vf[k - 2] * vdif[k - 2];
             Q3(s, j, k, 2) = Q3(s, j, k, 2) + (double)t2;
             Q3(s, j, k, 3) = Q3(s, j, k, 3) + (double)t3;
         }
