@@ -107,8 +107,11 @@ smac_status smac_get_state(smac_handle *h, double *q, double *anut, double *vnut
  * ref/smac2d.f:1395 together with the force/clift part of iterout at ref/smac2d.f:1407. */
 smac_status smac_step(smac_handle *h, smac_iter_stats *st);
 
-/* niter passes of smac_step without host round trips in between; stops early once
- * rhsnorm <= drop * (rhsnorm of the first pass) when drop > 0.  *done = passes made. */
+/* niter passes of smac_step.  With drop > 0 it stops early once rhsnorm <= drop * r0, r0 the
+ * rhsnorm of the case's first pass (nt = 1) when this handle made it, else of the first pass
+ * of this call; the stats are then read back after every pass (one small copy and a stream
+ * synchronisation).  With drop = 0 (and timing off) the passes are enqueued without host
+ * round trips and the stats are read once, after the last.  *done = passes made. */
 smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *last, int *done);
 
 /* Stats of the last pass made on h (by smac_step, smac_run or smac_batch_run). */

@@ -72,6 +72,7 @@ struct smac_handle {
     Walls walls = {};
     std::vector<BcRegion> regs;
     int nt = 0;
+    double rhs0 = 0.0;            // rhsnorm of pass nt = 1 when this handle made it (else 0)
     bool first_sa = true;
     bool keep_lhs = false;
     bool timing = false;
@@ -458,6 +459,7 @@ smac_status smac_set_state(smac_handle *h, const double *q, const double *anut, 
     st.clift = clift;
     HIP_TRY(hipMemcpy(h->st, &st, sizeof st, hipMemcpyHostToDevice));
     h->nt = nt;
+    h->rhs0 = 0.0;
     if (nt > 0) h->first_sa = false;
     return SMAC_OK;
 }
@@ -601,6 +603,7 @@ smac_status smac_step(smac_handle *h, smac_iter_stats *st)
     HIP_TRY(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     accumulate_timing(h);
+    if (h->nt == 1) h->rhs0 = s.rhsnorm;
     if (st) fill_stats(h, s, st);
     if (s.stall) return fail(SMAC_ERR_STALL, "LU-SGS band hand-over timed out at nt=" + std::to_string(h->nt));
     if (s.nonfinite) return fail(SMAC_ERR_NONFINITE, "residual or solution went non-finite at nt=" + std::to_string(h->nt));
@@ -634,7 +637,8 @@ smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *la
                 if (last) fill_stats(h, s, last);
                 return fail(SMAC_ERR_NONFINITE, "non-finite at nt=" + std::to_string(h->nt));
             }
-            if (first < 0.0) first = s.rhsnorm;
+            if (h->nt == 1) h->rhs0 = s.rhsnorm;
+            if (first < 0.0) first = h->rhs0 > 0.0 ? h->rhs0 : s.rhsnorm;
             if (drop > 0.0 && s.rhsnorm <= drop * first) { ++n; break; }
         }
     }
