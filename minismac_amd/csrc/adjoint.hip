@@ -21,11 +21,15 @@
 //  - k_adj_jtv + k_adj_wake_t: lambda -> F_q^T lambda, a gather over the 9 offsets plus the
 //    wake cut's far partner (jb,2).
 //  - the preconditioner: the transpose of the primal iteration's own solve
-//    P = W M^-1 (on non-target rows) + identity (on targets), M = (D+L) D^-1 (D+U) the LU-SGS
-//    operator of the converged state.  M^T = (D+U)^T D^-T (D+L)^T is again an LU-SGS
-//    operator, with D^T and the neighbour blocks transposed and shifted
+//    P = W M^-1 (on non-target rows) + identity (on targets).  For LU-SGS (impsch 2),
+//    M = (D+L) D^-1 (D+U) of the converged state; M^T = (D+U)^T D^-T (D+L)^T is again an
+//    LU-SGS operator, with D^T and the neighbour blocks transposed and shifted
 //    (bjm'(p) = bjp(p-1)^T, bkm'(p) = bkp(p-jm)^T, bjp'(p) = bjm(p+1)^T, bkp'(p) = bkm(p+jm)^T),
-//    so M^-T runs on the primal's own hyperplane band kernels (k_adj_transpose_blocks).
+//    so M^-T runs on the primal's own hyperplane band kernels (k_adj_transpose_blocks).  For
+//    line relaxation (impsch 1), M^-1 is the zebra sweeps of the blocks (xi then eta, wake
+//    injection only at the end, in W) and M^-T their exact transpose: the colour steps in
+//    reverse order, each a line solve of the transposed line system (the factorisation of
+//    the same transposed and shifted blocks) and a spread to the neighbouring lines.
 //  - the force gradient (k_adj_force_grad: CL_q, CD_q by dual numbers on the wall faces),
 //    the metric tangent of a shape direction (k_adj_metric_tangent, as the metric is formed:
 //    second-order differences, one-sided at the edges), the residual tangent at fixed q
@@ -853,6 +857,12 @@ void adj_build(AdjCtx &c)
                        c.q, c.jac);
     hipLaunchKernelGGL(k_adj_transpose_blocks, g3, dim3(256), 0, c.sm, d, c.blk[0], c.blk[1],
                        c.blk[2], c.blk[3], c.blk[4], c.tb[0], c.tb[1], c.tb[2], c.tb[3], c.tb[4]);
+    if (c.lr) {
+        launch_lr_factor3<double>(c.sm, d, c.njsp > 0, c.nksp > 0, c.blk[0], c.blk[1], c.blk[2],
+                                  c.blk[3], c.blk[4], (double *)c.lr_f.opx, (double *)c.lr_f.ope);
+        launch_lr_factor3<double>(c.sm, d, c.njsp > 0, c.nksp > 0, c.tb[0], c.tb[1], c.tb[2],
+                                  c.tb[3], c.tb[4], (double *)c.lr_t.opx, (double *)c.lr_t.ope);
+    }
     for (int o = 0; o < 2; ++o)
         hipLaunchKernelGGL(k_adj_axpby, dim3(nblk(3 * N)), dim3(256), 0, c.sm, 3 * N, 0.0,
                            (const double *)nullptr, 0.0, c.g[o], 0.0, (const double *)nullptr);
@@ -901,8 +911,12 @@ void adj_apply_prec_t(AdjCtx &c, const double *in, double *out)
         hipLaunchKernelGGL(k_adj_wake_wt, dim3((c.npair + 63) / 64), dim3(64), 0, c.sm, d,
                            c.npair, c.pairs, c.inv, in, c.t1);
     const WakePairs none{0, nullptr};
-    launch_lusgs3<double>(c.sm, d, c.ws, none, c.q, c.t2, c.t1, c.tb[0], c.tb[1], c.tb[2],
-                          c.tb[3], c.tb[4], nullptr, false);
+    if (c.lr)
+        launch_lrelax_scan_t<double>(c.sm, d, c.njsp, c.nksp, c.dp.underr, c.t1, c.t2, c.lr_t,
+                                     c.lrv);
+    else
+        launch_lusgs3<double>(c.sm, d, c.ws, none, c.q, c.t2, c.t1, c.tb[0], c.tb[1], c.tb[2],
+                              c.tb[3], c.tb[4], nullptr, false);
     hipLaunchKernelGGL(k_adj_merge_targets, dim3(nblk(3 * N)), dim3(256), 0, c.sm, N, c.rtype,
                        in, c.t2, out);
 }
@@ -915,8 +929,13 @@ void adj_apply_prec(AdjCtx &c, const double *in, double *out)
     hipLaunchKernelGGL(k_adj_mask_targets, dim3(nblk(3 * N)), dim3(256), 0, c.sm, N, c.rtype, 0,
                        in, c.t1);
     const WakePairs none{0, nullptr};
-    launch_lusgs3<double>(c.sm, d, c.ws, none, c.q, out, c.t1, c.blk[0], c.blk[1], c.blk[2],
-                          c.blk[3], c.blk[4], nullptr, false);
+    if (c.lr)
+        launch_lrelax_scan<double>(c.sm, d, c.dp, c.njsp, c.nksp, 1, true, none, c.q, out, c.t1,
+                                   c.blk[0], c.blk[1], c.blk[2], c.blk[3], c.blk[4], c.lr_f,
+                                   false);
+    else
+        launch_lusgs3<double>(c.sm, d, c.ws, none, c.q, out, c.t1, c.blk[0], c.blk[1], c.blk[2],
+                              c.blk[3], c.blk[4], nullptr, false);
     if (c.npair > 0)
         hipLaunchKernelGGL(k_adj_wake_w, dim3((c.npair + 63) / 64), dim3(64), 0, c.sm, d, c.npair,
                            c.pairs, out, in, out);

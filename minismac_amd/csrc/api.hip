@@ -328,10 +328,12 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
     if (p->impsch == 1) {
         char *lb = nullptr;
         TRY(dalloc(h, &lb, (size_t)lr_scan_bytes(h->d)));
-        h->lrs.op = lb;
-        h->lrs.s_t = (double *)(lb + 45 * N * 8 + 256);
+        h->lrs.opx = lb;
+        h->lrs.ope = lb + 45 * N * 8 + 256;
+        h->lrs.s_t = (double *)(lb + 2 * (45 * N * 8 + 256));
         h->lrs.dq_t = h->lrs.s_t + 3 * N;
         h->lrs.sa_op = (double *)((char *)(h->lrs.dq_t + 3 * N) + 256);
+        h->lrs.zebra = p->line_order == SMAC_ORDER_ZEBRA;
         h->lr_solve = scan_only ? 1 : 0;
     }
     h->w.nred = 512;
@@ -766,7 +768,24 @@ static smac_status adj_prepare(smac_handle *h, int m)
             (rc = dalloc(h, &c.part, 512 * (m + 2))) || (rc = dalloc(h, &c.red, m + 2)) ||
             (rc = dalloc(h, &c.red2, m + 2)))
             return rc;
-        if (!h->adj_ws.lhs) {
+        c.lr = h->p.impsch == 1;
+        if (c.lr) {
+            // the factored lines of M and M^T (FP64) and the transposed sweeps' vectors
+            char *f = nullptr, *t = nullptr;
+            if ((rc = dalloc(h, &f, 2 * 45 * N * 8)) || (rc = dalloc(h, &t, 2 * 45 * N * 8)))
+                return rc;
+            c.lr_f.opx = f;
+            c.lr_f.ope = f + 45 * N * 8;
+            c.lr_t.opx = t;
+            c.lr_t.ope = t + 45 * N * 8;
+            double *sv = nullptr;
+            if ((rc = dalloc(h, &sv, 6 * N))) return rc;
+            c.lr_f.s_t = c.lr_t.s_t = sv;
+            c.lr_f.dq_t = c.lr_t.dq_t = sv + 3 * N;
+            double **lv[] = {&c.lrv.a, &c.lrv.z, &c.lrv.at, &c.lrv.st, &c.lrv.zt};
+            for (double **pp : lv)
+                if ((rc = dalloc(h, pp, 3 * N))) return rc;
+        } else if (!h->adj_ws.lhs) {
             h->adj_ws.ring = 1;
             rc = alloc_one_ws(h, 64, 3, h->adj_ws);
             if (rc) return rc;
@@ -784,6 +803,9 @@ static smac_status adj_prepare(smac_handle *h, int m)
     c.beta = h->dp.beta; c.cdis = h->dp.cdis; c.vnu = h->dp.vnu; c.pout = h->dp.pout;
     c.alpha = h->dp.alpha; c.chord = h->dp.chord; c.ivis = h->dp.ivis;
     c.m = m;
+    c.njsp = h->p.njsp;
+    c.nksp = h->p.nksp;
+    c.dp = h->dp;
     c.npair = h->wake.n;
     c.pairs = h->wake.pairs;
     return SMAC_OK;

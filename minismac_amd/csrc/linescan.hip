@@ -27,6 +27,10 @@
 #include "smac_dev.h"
 #include "smac_kernels.h"
 
+// This file's arithmetic is reassociated anyway, so multiply-adds may fuse here (the library
+// is otherwise built without contraction, which keeps the reference-order paths bit-exact).
+#pragma clang fp contract(fast)
+
 namespace smac {
 
 // operand planes per point: factored diagonal (B, or 1/b for the scalar system), M, -C,
@@ -92,129 +96,185 @@ SMAC_DEV void aff_shfl_down(Aff<T, NEQ> &o, const Aff<T, NEQ> &a, int d)
 }
 
 // ------------------------------------------------------------------------------------------
-// factorisation (3x3): btlu3j / btlu3k (ref/btlu3.f:52-624) for every line at once, one
-// thread per line, writing the line-major operand planes.  lm / lp: blocks along the line
-// (bjm/bjp for xi lines, bkm/bkp for eta lines); om / op: blocks to the neighbouring lines.
+// factorisation: the block Thomas recurrence of btlu3j / btlu3k (ref/btlu3.f:52-624) and of
+// scatri (ref/lreltur.f:337-359), for every line of both directions in one launch, one
+// wavefront per line.  Waves [0, nx) take the xi lines (k = const: contiguous along the line),
+// waves [nx, nx + ne) the eta lines (j = const: stride jm).  The recurrence
+//     T_i = b_i - a_i C_(i-1),   C_i = T_i^-1 c_i            (a_0 = 0, c_(len-1) = 0)
+// is sequential along the line, so the wave walks it in chunks of 64 points: lane u loads
+// point c0+u's blocks (coalesced for xi lines; for eta lines each lane reads its own cache
+// line, which the neighbouring lines' waves share through L2), the blocks are staged in LDS,
+// the next chunk's loads are issued, and then every lane steps through the chunk's points
+// reading each point's blocks from one LDS address (a broadcast) -- the recurrence runs
+// redundantly in all lanes, which costs nothing in a wave that has no other work.  A step
+// forms T_i^-1 c_i by the adjugate (one reciprocal on the chain) and writes C_i to LDS; after
+// the chunk lane u redoes its own point's pivot from C_(u-1), forms T_i^-1 and
+// M_i = -T_i^-1 a_i, and stores the operand planes line-major with coalesced stores.  The blocks to the neighbouring lines are copied
+// through unchanged.  (The reference keeps T_i as LU factors; the explicit inverse makes
+// the sweeps' T_i^-1 r_i a parallel product.)
 
-template <typename T>
-SMAC_DEV void lu3s(T b[9])
+// 1/x: the hardware reciprocal refined by Newton-Raphson steps r <- r + r (1 - x r), each of
+// which doubles the number of correct bits (two for FP64, one for FP32)
+SMAC_DEV double rcp_nr(double x)
 {
-    b[0] = (T)1. / b[0];
-    b[3] = b[3] * b[0];
-    b[6] = b[6] * b[0];
-    b[4] = (T)1. / (b[4] - b[1] * b[3]);
-    b[7] = (b[7] - b[1] * b[6]) * b[4];
-    b[5] = b[5] - b[2] * b[3];
-    b[8] = (T)1. / (b[8] - b[2] * b[6] - b[7] * b[5]);
+    double r = __builtin_amdgcn_rcp(x);
+    double e = 1.0 - x * r;
+    r = r + r * e;
+    e = 1.0 - x * r;
+    return r + r * e;
+}
+SMAC_DEV float rcp_nr(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    return r + r * (1.0f - x * r);
 }
 
+// adj(T) (column-major, entry (r, c) at r + 3c) and det(T) of a 3x3 T
 template <typename T>
-SMAC_DEV void csolve3s(const T b[9], T c[9])
+SMAC_DEV T adj3(const T t[9], T adj[9])
 {
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        T d1 = b[0] * c[3 * n];
-        T d2 = b[4] * (c[1 + 3 * n] - b[1] * d1);
-        T d3 = b[8] * (c[2 + 3 * n] - b[2] * d1 - b[5] * d2);
-        c[2 + 3 * n] = d3;
-        c[1 + 3 * n] = d2 - b[7] * c[2 + 3 * n];
-        c[3 * n] = d1 - b[6] * c[2 + 3 * n] - b[3] * c[1 + 3 * n];
-    }
+    adj[0] = t[4] * t[8] - t[7] * t[5];
+    adj[1] = t[7] * t[2] - t[1] * t[8];
+    adj[2] = t[1] * t[5] - t[4] * t[2];
+    adj[3] = t[6] * t[5] - t[3] * t[8];
+    adj[4] = t[0] * t[8] - t[6] * t[2];
+    adj[5] = t[3] * t[2] - t[0] * t[5];
+    adj[6] = t[3] * t[7] - t[6] * t[4];
+    adj[7] = t[6] * t[1] - t[0] * t[7];
+    adj[8] = t[0] * t[4] - t[3] * t[1];
+    return t[0] * adj[0] + t[3] * adj[1] + t[6] * adj[2];
 }
 
-template <typename LT>
-__global__ void __launch_bounds__(64)
-k_lr_factor3(Dims d, int eta, const LT *__restrict__ btc, const LT *__restrict__ lm,
-             const LT *__restrict__ lp, const LT *__restrict__ om, const LT *__restrict__ op,
-             LT *__restrict__ out)
+// out = x y (NEQ x NEQ, column-major)
+template <typename T, int NEQ>
+SMAC_DEV void mm(const T *x, const T *y, T *out)
 {
-    const long N = d.N;
-    const int nline = eta ? d.jm : d.km, len = eta ? d.km : d.jm;
-    const long stride = eta ? d.jm : 1, lstride = eta ? 1 : d.jm;
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nline) return;
-    LT cprev[9];
-    for (int i = 0; i < len; ++i) {
-        const long p = lstride * l + stride * i;     // natural layout
-        const long o = (long)l * len + i;            // line-major
-        LT bb[9], mm[9], cc[9], t[9];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) bb[e] = btc[e * N + p];
-        if (i > 0) {
-            LT aa[9];
+    for (int r = 0; r < NEQ; ++r)
 #pragma unroll
-            for (int e = 0; e < 9; ++e) aa[e] = lm[e * N + p];
-            // b <- b - a C(i-1)
+        for (int c = 0; c < NEQ; ++c) {
+            T acc = x[r] * y[NEQ * c];
 #pragma unroll
-            for (int n = 0; n < 3; ++n)
-#pragma unroll
-                for (int m = 0; m < 3; ++m)
-                    bb[m + 3 * n] = bb[m + 3 * n] - aa[m] * cprev[3 * n] -
-                                    aa[m + 3] * cprev[1 + 3 * n] - aa[m + 6] * cprev[2 + 3 * n];
-            lu3s(bb);
-#pragma unroll
-            for (int e = 0; e < 9; ++e) mm[e] = -aa[e];
-            csolve3s(bb, mm);                        // M = -B^-1 a
-        } else {
-            lu3s(bb);
-#pragma unroll
-            for (int e = 0; e < 9; ++e) mm[e] = (LT)0;
+            for (int k = 1; k < NEQ; ++k) acc = acc + x[r + NEQ * k] * y[k + NEQ * c];
+            out[r + NEQ * c] = acc;
         }
-        if (i < len - 1) {
+}
+
+constexpr int FW = 64;   // points per chunk = lanes of the wave
+
+template <typename LT, int NEQ>
+__global__ void __launch_bounds__(FW)
+k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ bjm,
+            const LT *__restrict__ bjp, const LT *__restrict__ bkm, const LT *__restrict__ bkp,
+            LT *__restrict__ opx, LT *__restrict__ ope)
+{
+    constexpr int NB = NEQ * NEQ;
+    const int lane = threadIdx.x;
+    const bool eta = (int)blockIdx.x >= nx;
+    // the xi lines are the longer chains: their waves win issue slots on a shared SIMD
+    if (!eta) __builtin_amdgcn_s_setprio(2);
+    const int l = eta ? (int)blockIdx.x - nx : (int)blockIdx.x;
+    const int len = eta ? d.km : d.jm;
+    const long N = d.N;
+    const long stride = eta ? d.jm : 1, base = eta ? (long)l : (long)l * d.jm;
+    const LT *lm = eta ? bkm : bjm, *lp = eta ? bkp : bjp, *om = eta ? bjm : bkm,
+             *opn = eta ? bjp : bkp;
+    LT *out = eta ? ope : opx;
+    __shared__ LT sh[FW][3 * NB + 1];   // a, b, c of the chunk's points
+    __shared__ LT res[FW][NB + 1];      // C of the chunk's points
+    LT ra[NB], rb[NB], rc[NB], rom[NB], rop[NB];
+    auto load = [&](int c0) {
+        const int i = c0 + lane;
+        if (i < len) {
+            const long p = base + stride * i;
 #pragma unroll
-            for (int e = 0; e < 9; ++e) cc[e] = lp[e * N + p];
-            csolve3s(bb, cc);                        // C = B^-1 c
-#pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                cprev[e] = cc[e];
-                cc[e] = -cc[e];
+            for (int e = 0; e < NB; ++e) {
+                rb[e] = btc[e * N + p];
+                ra[e] = i > 0 ? lm[e * N + p] : (LT)0;
+                rc[e] = i < len - 1 ? lp[e * N + p] : (LT)0;
+                rom[e] = om[e * N + p];
+                rop[e] = opn[e * N + p];
             }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 9; ++e) cc[e] = (LT)0;
         }
+    };
+    // T = b - a C;  det(T) and adj(T) (T^-1 = adj / det)
+    auto pivot = [&](const LT *a, LT *t, const LT *C, LT *ad) -> LT {
 #pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            out[e * N + o] = bb[e];
-            out[(9 + e) * N + o] = mm[e];
-            out[(18 + e) * N + o] = cc[e];
+        for (int cc = 0; cc < NEQ; ++cc)
+#pragma unroll
+            for (int r = 0; r < NEQ; ++r)
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                    t[r + NEQ * cc] = t[r + NEQ * cc] - a[r + NEQ * k] * C[k + NEQ * cc];
+        if constexpr (NEQ == 3) return adj3(t, ad);
+        ad[0] = (LT)1;
+        return t[0];
+    };
+    LT C[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) C[e] = (LT)0;
+    load(0);
+    for (int c0 = 0; c0 < len; c0 += FW) {
+        const int i = c0 + lane;
+        const long o = (long)l * len + i;
+        LT oa[NB], ob[NB], cin[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            oa[e] = ra[e];
+            ob[e] = rb[e];
+            cin[e] = C[e];
         }
+        if (i < len) {
 #pragma unroll
-        for (int e = 0; e < 9; ++e) t[e] = om[e * N + p];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) out[(27 + e) * N + o] = t[e];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) t[e] = op[e * N + p];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) out[(36 + e) * N + o] = t[e];
-    }
-}
-
-// factorisation of the SA's scalar xi lines (scatri, ref/lreltur.f:337-359): 1/b, -a/b, -C,
-// and the k-neighbour coefficients skm/skp, all in the natural layout (= line-major for xi)
-__global__ void __launch_bounds__(64)
-k_lr_factor1(Dims d, const double *__restrict__ sjm, const double *__restrict__ stc,
-             const double *__restrict__ sjp, const double *__restrict__ skm,
-             const double *__restrict__ skp, double *__restrict__ out)
-{
-    const long N = d.N;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= d.km) return;
-    double cprev = 0.0;
-    for (int j = 0; j < d.jm; ++j) {
-        const long p = P(d, j, k);
-        const double a = sjm[p], b = stc[p];
-        const double z = j > 0 ? 1. / (b - a * cprev) : 1. / b;
-        double c = 0.0;
-        if (j < d.jm - 1) {
-            c = sjp[p] * z;
-            cprev = c;
+            for (int e = 0; e < NB; ++e) {
+                sh[lane][e] = ra[e];
+                sh[lane][NB + e] = rb[e];
+                sh[lane][2 * NB + e] = rc[e];
+                out[(3 * NB + e) * N + o] = rom[e];
+                out[(4 * NB + e) * N + o] = rop[e];
+            }
         }
-        out[p] = z;
-        out[N + p] = j > 0 ? -a * z : 0.0;
-        out[2 * N + p] = -c;
-        out[3 * N + p] = skm[p];
-        out[4 * N + p] = skp[p];
+        __syncthreads();
+        if (c0 + FW < len) load(c0 + FW);
+        const int nv = len - c0 < FW ? len - c0 : FW;
+        // the recurrence, in every lane: only C goes on to the next point
+#pragma unroll 2
+        for (int u = 0; u < nv; ++u) {
+            LT a[NB], t[NB], c[NB], ad[NB], ac[NB];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                a[e] = sh[u][e];
+                t[e] = sh[u][NB + e];
+                c[e] = sh[u][2 * NB + e];
+            }
+            const LT det = pivot(a, t, C, ad);
+            mm<LT, NEQ>(ad, c, ac);
+            const LT rd = rcp_nr(det);
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                C[e] = ac[e] * rd;
+                res[u][e] = C[e];
+            }
+        }
+        __syncthreads();
+        // each lane redoes its own point's pivot from the C entering it (the same operations
+        // as the recurrence, so the same values) and forms T^-1 and M = -T^-1 a
+        if (i < len) {
+            LT cp[NB], ad[NB], bi[NB], m[NB];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) cp[e] = lane > 0 ? res[lane - 1][e] : cin[e];
+            const LT rd = rcp_nr(pivot(oa, ob, cp, ad));
+#pragma unroll
+            for (int e = 0; e < NB; ++e) bi[e] = ad[e] * rd;
+            mm<LT, NEQ>(bi, oa, m);
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                out[e * N + o] = bi[e];
+                out[(NB + e) * N + o] = -m[e];
+                out[(2 * NB + e) * N + o] = -res[lane][e];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -222,12 +282,13 @@ k_lr_factor1(Dims d, const double *__restrict__ sjm, const double *__restrict__ 
 // the sweep: lines count of them, line l = (first + idx*step) mod nline.  SEQ: one
 // workgroup walks them in order; otherwise one workgroup per line (lines of one colour).
 // s, dq: NEQ planes, line-major; op: the factor kernel's planes.  assign: dq = x (the SA's
-// lreltur); otherwise dq += underr (x - dq) (ref/lrelax.f:377-381).
+// lreltur); otherwise dq += underr (x - dq) (ref/lrelax.f:377-381).  couple = 0 leaves out
+// the neighbouring lines' terms: a plain solve of each line (the transposed sweeps).
 
 template <typename LT, int NEQ, int NT, int PP, bool SEQ>
 __global__ void __launch_bounds__(NT)
 k_lr_sweep(int nline, int len, long N, int count, int first, int step, double underr,
-           int assign, double *__restrict__ dq, const double *__restrict__ s,
+           int assign, int couple, double *__restrict__ dq, const double *__restrict__ s,
            const LT *__restrict__ op)
 {
     using AT = LT;
@@ -241,7 +302,7 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
     for (int idx = SEQ ? 0 : (int)blockIdx.x; idx < count; idx += SEQ ? 1 : (int)gridDim.x) {
         int l = (first + idx * step) % nline;
         if (l < 0) l += nline;
-        const bool hm = l > 0, hp = l < nline - 1;
+        const bool hm = couple && l > 0, hp = couple && l < nline - 1;
         const bool regm = SEQ && lprev == l - 1, regp = SEQ && lprev == l + 1;
         Aff<AT, NEQ> mp[PP], agg;
         AT cm[PP][NB];
@@ -282,19 +343,19 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                         r[m] = r[m] - b;
                     }
                 }
-                if constexpr (NEQ == 3) {
-                    AT bb[9];
+                {
+                    AT bi[NB], rr[NEQ];
 #pragma unroll
-                    for (int e = 0; e < 9; ++e) bb[e] = pB[e * N + o];
-                    const AT t1 = (AT)r[0], t2 = (AT)r[1], t3 = (AT)r[2];
-                    const AT d1 = bb[0] * t1;
-                    const AT d2 = bb[4] * (t2 - bb[1] * d1);
-                    const AT d3 = bb[8] * (t3 - bb[2] * d1 - bb[5] * d2);
-                    mp[u].v[2] = d3;
-                    mp[u].v[1] = d2 - bb[7] * d3;
-                    mp[u].v[0] = d1 - bb[6] * d3 - bb[3] * mp[u].v[1];
-                } else {
-                    mp[u].v[0] = pB[o] * (AT)r[0];
+                    for (int e = 0; e < NB; ++e) bi[e] = pB[e * N + o];
+#pragma unroll
+                    for (int n = 0; n < NEQ; ++n) rr[n] = (AT)r[n];
+#pragma unroll
+                    for (int m = 0; m < NEQ; ++m) {
+                        AT acc = bi[m] * rr[0];
+#pragma unroll
+                        for (int n = 1; n < NEQ; ++n) acc = acc + bi[m + NEQ * n] * rr[n];
+                        mp[u].v[m] = acc;
+                    }
                 }
 #pragma unroll
                 for (int e = 0; e < NB; ++e) {
@@ -485,12 +546,12 @@ __global__ void k_zero(double *__restrict__ a, long n)
 template <typename LT, int NEQ, bool SEQ>
 static void sweep_launch(hipStream_t sm, int nline, int len, long N, int count, int first,
                          int step, double underr, int assign, double *dq, const double *s,
-                         const LT *op)
+                         const LT *op, int couple = 1)
 {
     const dim3 g(SEQ ? 1 : count);
 #define SW(NT, PPV)                                                                            \
     hipLaunchKernelGGL((k_lr_sweep<LT, NEQ, NT, PPV, SEQ>), g, dim3(NT), 0, sm, nline, len, N, \
-                       count, first, step, underr, assign, dq, s, op)
+                       count, first, step, underr, assign, couple, dq, s, op)
     if (len <= 256) SW(256, 1);
     else if (len <= 512) SW(512, 1);
     else if (len <= 1024) SW(512, 2);
@@ -516,34 +577,42 @@ static void sweep(hipStream_t sm, int nline, int len, long N, int nsweep, bool z
 
 long lr_scan_bytes(const Dims &d)
 {
-    // 45 operand planes (FP64 size) + eta-major s and dq + the SA's 5 scalar planes
-    return (long)(lr_planes<3>() + 6 + lr_planes<1>()) * d.N * 8 + 4 * 256;
+    // 45 operand planes (FP64 size) per direction + eta-major s and dq + the SA's 5 planes
+    return (long)(2 * lr_planes<3>() + 6 + lr_planes<1>()) * d.N * 8 + 4 * 256;
+}
+
+template <typename LT>
+void launch_lr_factor3(hipStream_t sm, const Dims &d, bool xi, bool eta, const LT *btc,
+                       const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *opx,
+                       LT *ope)
+{
+    const int nx = xi ? d.km : 0, ne = eta ? d.jm : 0;
+    if (nx + ne > 0)
+        hipLaunchKernelGGL((k_lr_factor<LT, 3>), dim3(nx + ne), dim3(FW), 0, sm, d, nx, btc, bjm,
+                           bjp, bkm, bkp, opx, ope);
 }
 
 template <typename LT>
 void launch_lrelax_scan(hipStream_t sm, const Dims &d, const DevParams &prm, int njsp, int nksp,
                         int nbcimp, bool zebra, const WakePairs &w, const double *q, double *dq,
                         double *s, const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm,
-                        const LT *bkp, const LrScan &ws)
+                        const LT *bkp, const LrScan &ws, bool factor)
 {
     const long N = d.N;
-    LT *op = (LT *)ws.op;
+    LT *opx = (LT *)ws.opx, *ope = (LT *)ws.ope;
     hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, sm, dq, 3 * N);
-    if (njsp > 0) {
-        hipLaunchKernelGGL(k_lr_factor3<LT>, dim3((d.km + 63) / 64), dim3(64), 0, sm, d, 0, btc,
-                           bjm, bjp, bkm, bkp, op);
-        for (int nj = 1; nj <= njsp; ++nj) {
-            sweep<LT, 3>(sm, d.km, d.jm, N, nj, zebra, prm.underr, dq, s, op);
-            if ((nj / nbcimp) * nbcimp == nj) launch_wake(sm, d, 3, w, q, dq, s);
-        }
+    // both directions' factorisations depend on the blocks only: one launch
+    if (factor)
+        launch_lr_factor3<LT>(sm, d, njsp > 0, nksp > 0, btc, bjm, bjp, bkm, bkp, opx, ope);
+    for (int nj = 1; nj <= njsp; ++nj) {
+        sweep<LT, 3>(sm, d.km, d.jm, N, nj, zebra, prm.underr, dq, s, opx);
+        if ((nj / nbcimp) * nbcimp == nj) launch_wake(sm, d, 3, w, q, dq, s);
     }
     if (nksp > 0) {
-        hipLaunchKernelGGL(k_lr_factor3<LT>, dim3((d.jm + 63) / 64), dim3(64), 0, sm, d, 1, btc,
-                           bkm, bkp, bjm, bjp, op);
         transpose(sm, s, ws.s_t, d.km, d.jm, N, 3);
         transpose(sm, dq, ws.dq_t, d.km, d.jm, N, 3);
         for (int nk = 1; nk <= nksp; ++nk) {
-            sweep<LT, 3>(sm, d.jm, d.km, N, nk, zebra, prm.underr, ws.dq_t, ws.s_t, op);
+            sweep<LT, 3>(sm, d.jm, d.km, N, nk, zebra, prm.underr, ws.dq_t, ws.s_t, ope);
             if ((nk / nbcimp) * nbcimp == nk && w.n > 0)
                 hipLaunchKernelGGL(k_wake_eta, dim3((w.n + 63) / 64), dim3(64), 0, sm, N, d.jm,
                                    d.km, 3, w.n, w.base, w.pairs, q, ws.dq_t, ws.s_t);
@@ -553,36 +622,136 @@ void launch_lrelax_scan(hipStream_t sm, const Dims &d, const DevParams &prm, int
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// the transpose of the zebra sweeps (the adjoint's preconditioner).  The sweeps map the
+// right-hand side s to dq linearly (dq starts at zero; one colour step replaces dq_l on the
+// lines of its colour by (1-w) dq_l + w T_l^-1 (s_l - Om_l dq_(l-1) - Op_l dq_(l+1))).  Their
+// transpose runs the colour steps in reverse order, each transposed: with a the adjoint of
+// dq and z_l = T_l^-T a_l on the colour's lines (a plain line solve of the transposed line
+// system, whose factorisation is that of the transposed and shifted blocks),
+//     sbar_l += w z_l,   a_l <- (1-w) a_l                 (lines of the colour)
+//     a_m -= w (Om'_m z_(m-1) + Op'_m z_(m+1))            (the other lines)
+// where Om'_m = (Op_(m-1))^T and Op'_m = (Om_(m+1))^T are again the transposed system's
+// blocks to the neighbouring lines at the point.
+
+template <typename LT>
+__global__ void __launch_bounds__(256)
+k_lr_tspread(int nline, int len, long N, int par, double w, double *__restrict__ a,
+             double *__restrict__ sbar, const double *__restrict__ z, const LT *__restrict__ op)
+{
+    const LT *pOm = op + 27 * N, *pOp = op + 36 * N;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < N;
+         o += (long)gridDim.x * blockDim.x) {
+        const int l = (int)(o / len);
+        if ((l & 1) == par) {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                sbar[n * N + o] += w * z[n * N + o];
+                a[n * N + o] *= 1.0 - w;
+            }
+            continue;
+        }
+        double acc[3] = {0.0, 0.0, 0.0};
+        if (l > 0) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    acc[m] += (double)pOm[(m + 3 * k) * N + o] * z[k * N + o - len];
+        }
+        if (l < nline - 1) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    acc[m] += (double)pOp[(m + 3 * k) * N + o] * z[k * N + o + len];
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) a[m * N + o] -= w * acc[m];
+    }
+}
+
+__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+template <typename LT>
+static void tsweep(hipStream_t sm, int nline, int len, long N, int nsweep, double w, double *a,
+                   double *sbar, double *z, const LT *op)
+{
+    const unsigned g = (unsigned)((N + 255) / 256 < 2048 ? (N + 255) / 256 : 2048);
+    for (int sw = 0; sw < nsweep; ++sw)
+        for (int par = 1; par >= 0; --par) {
+            const int count = par ? nline / 2 : (nline + 1) / 2;
+            if (count > 0)
+                sweep_launch<LT, 3, false>(sm, nline, len, N, count, par, 2, 1.0, 1, z, a, op, 0);
+            hipLaunchKernelGGL(k_lr_tspread<LT>, dim3(g), dim3(256), 0, sm, nline, len, N, par, w,
+                               a, sbar, (const double *)z, op);
+        }
+}
+
+template <typename LT>
+void launch_lrelax_scan_t(hipStream_t sm, const Dims &d, int njsp, int nksp, double underr,
+                          const double *in, double *out, const LrScan &wt, const LrTWork &v)
+{
+    const long N = d.N;
+    const unsigned g = (unsigned)((3 * N + 255) / 256 < 2048 ? (3 * N + 255) / 256 : 2048);
+    hipLaunchKernelGGL(k_copy, dim3(g), dim3(256), 0, sm, in, v.a, 3 * N);
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, sm, out, 3 * N);
+    // the eta sweeps came last in the primal: their transposes come first
+    if (nksp > 0) {
+        transpose(sm, v.a, v.at, d.km, d.jm, N, 3);
+        hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, sm, v.st, 3 * N);
+        tsweep<LT>(sm, d.jm, d.km, N, nksp, underr, v.at, v.st, v.zt, (const LT *)wt.ope);
+        transpose(sm, v.st, out, d.jm, d.km, N, 3);
+        transpose(sm, v.at, v.a, d.jm, d.km, N, 3);
+    }
+    if (njsp > 0) tsweep<LT>(sm, d.km, d.jm, N, njsp, underr, v.a, out, v.z, (const LT *)wt.opx);
+}
+
 void launch_sa_lreltur_scan(hipStream_t sm, const Dims &d, int ntjsp, const WakePairs &w,
                             SaArrays &a, const LrScan &ws)
 {
     // lreltur (ref/lreltur.f:52-252): ds = 0; ntjsp passes of the k lines in ascending order
-    // followed by a re-solve of line k = 1, each pass ending with the wake injection
+    // followed by a re-solve of line k = 1, each pass ending with the wake injection.  With
+    // the zebra order (that of the mean flow) a pass takes the even lines, the odd lines, and
+    // then line k = 1 again.
     hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, sm, a.ds, d.N);
-    hipLaunchKernelGGL(k_lr_factor1, dim3((d.km + 63) / 64), dim3(64), 0, sm, d, a.sjm, a.stc,
-                       a.sjp, a.skm, a.skp, ws.sa_op);
+    hipLaunchKernelGGL((k_lr_factor<double, 1>), dim3(d.km), dim3(FW), 0, sm, d, d.km, a.stc,
+                       a.sjm, a.sjp, a.skm, a.skp, ws.sa_op, (double *)nullptr);
     for (int pass = 1; pass <= ntjsp; ++pass) {
-        const dim3 g(1);
-#define SW1(NT, PPV)                                                                           \
-    hipLaunchKernelGGL((k_lr_sweep<double, 1, NT, PPV, true>), g, dim3(NT), 0, sm, d.km, d.jm,  \
-                       d.N, d.km + 1, 0, 1, 1.0, 1, a.ds, a.sf, ws.sa_op)
-        if (d.jm <= 256) SW1(256, 1);
-        else if (d.jm <= 512) SW1(512, 1);
-        else if (d.jm <= 1024) SW1(512, 2);
-        else SW1(1024, 2);
-#undef SW1
+        if (ws.zebra) {
+            sweep_launch<double, 1, false>(sm, d.km, d.jm, d.N, (d.km + 1) / 2, 0, 2, 1.0, 1,
+                                           a.ds, a.sf, ws.sa_op);
+            sweep_launch<double, 1, false>(sm, d.km, d.jm, d.N, d.km / 2, 1, 2, 1.0, 1, a.ds,
+                                           a.sf, ws.sa_op);
+            sweep_launch<double, 1, false>(sm, d.km, d.jm, d.N, 1, 0, 1, 1.0, 1, a.ds, a.sf,
+                                           ws.sa_op);
+        } else {
+            sweep_launch<double, 1, true>(sm, d.km, d.jm, d.N, d.km + 1, 0, 1, 1.0, 1, a.ds, a.sf,
+                                          ws.sa_op);
+        }
         launch_wake(sm, d, 1, w, a.anut, a.ds, a.sf);
     }
 }
 
+template void launch_lr_factor3<double>(hipStream_t, const Dims &, bool, bool, const double *,
+                                        const double *, const double *, const double *,
+                                        const double *, double *, double *);
+template void launch_lrelax_scan_t<double>(hipStream_t, const Dims &, int, int, double,
+                                           const double *, double *, const LrScan &,
+                                           const LrTWork &);
 template void launch_lrelax_scan<double>(hipStream_t, const Dims &, const DevParams &, int, int,
                                          int, bool, const WakePairs &, const double *, double *,
                                          double *, const double *, const double *,
                                          const double *, const double *, const double *,
-                                         const LrScan &);
+                                         const LrScan &, bool);
 template void launch_lrelax_scan<float>(hipStream_t, const Dims &, const DevParams &, int, int,
                                         int, bool, const WakePairs &, const double *, double *,
                                         double *, const float *, const float *, const float *,
-                                        const float *, const float *, const LrScan &);
+                                        const float *, const float *, const LrScan &, bool);
 
 }  // namespace smac

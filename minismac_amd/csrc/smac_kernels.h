@@ -53,16 +53,31 @@ void launch_wake(hipStream_t, const Dims &, int neqs, const WakePairs &, const d
 // linescan.hip: line relaxation with each line's solve parallel inside the line (prefix scans
 // of the substitutions' affine maps), in the reference's line order or the zebra order
 struct LrScan {
-    void *op;            // 45 line-major operand planes (LT)
+    void *opx, *ope;     // 45 line-major operand planes (LT) of the xi and of the eta lines
     double *s_t, *dq_t;  // eta-major s and dq (3 planes each)
     double *sa_op;       // 5 planes of the SA's scalar line system
+    bool zebra;          // the SA's lines in the zebra order too
 };
 long lr_scan_bytes(const Dims &);
+// the line factorisations of both directions (either may be skipped) into opx / ope
+template <typename LT>
+void launch_lr_factor3(hipStream_t, const Dims &, bool xi, bool eta, const LT *btc,
+                       const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *opx,
+                       LT *ope);
+// factor = false: the operand planes in the workspace are already those of these blocks
 template <typename LT>
 void launch_lrelax_scan(hipStream_t, const Dims &, const DevParams &, int njsp, int nksp,
                         int nbcimp, bool zebra, const WakePairs &, const double *q, double *dq,
                         double *s, const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm,
-                        const LT *bkp, const LrScan &);
+                        const LT *bkp, const LrScan &, bool factor = true);
+// out = L^T in, L the zebra sweeps (no wake injection) of the blocks factored in wt as their
+// transposed and shifted system; v: work vectors of 3 planes each
+struct LrTWork {
+    double *a, *z, *at, *st, *zt;
+};
+template <typename LT>
+void launch_lrelax_scan_t(hipStream_t, const Dims &, int njsp, int nksp, double underr,
+                          const double *in, double *out, const LrScan &wt, const LrTWork &v);
 
 // lusgs.hip: LU-SGS with hyperplane (j+k = const) sweeps (ref/lusgs.f, ref/ilusol.f).
 // Workspace in the sheared layout: point (j,k) at (j+k+16)*ks + k of every plane, planes
@@ -158,6 +173,11 @@ struct AdjCtx {
     int npair;
     const int *pairs, *inv;      // wake pairs (0-based) and, per pair, the pair whose partner
                                  // is its target
+    int lr;                      // preconditioner: 1 the zebra line relaxation, 0 LU-SGS
+    int njsp, nksp;
+    DevParams dp;
+    LrScan lr_f, lr_t;           // factored lines of M and of M^T (lr = 1)
+    LrTWork lrv;
     double *jac, *tb[5], *lam[2], *g[2], *fc, *fa, *V, *w, *z, *t1, *t2, *t1b;
     double *dmet, *ddj, *dxy, *fb, *fpart;
     double *part, *red, *red2;

@@ -124,6 +124,34 @@ def test_adjoint_update_is_the_primal_update_and_its_transpose():
             assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
+def test_adjoint_line_relaxation_update_and_its_transpose():
+    """With line relaxation the preconditioner is the zebra sweeps of the state's blocks with
+    the wake injection at the end, and its exact transpose.  With one xi sweep and no eta
+    sweep that is the primal's own pass, so P F(q) is the pass's dq; with the workload's
+    sweeps (two each way, under-relaxed) P and P^T agree in dot products."""
+    rng = np.random.default_rng(5)
+    case = _steady_case(njsp=1, nksp=0)
+    S = adjoint_cpu.Steady(case)
+    with Solver(case) as g:
+        st, _ = g.run(40)
+        q, _, _ = g.get_state()
+        F = S.F(q, st["cl"])
+        g.adjoint((), maxit=2, tol=1e-3)
+        pf = g.adjoint_apply(3, F)
+        g.set_state(q=q, nt=41, resmax0=1.0, clift=st["cl"])
+        g.step()
+        dq = g.array("dq")
+    assert np.abs(pf - dq).max() <= 1e-12 * np.abs(dq).max()
+    with Solver(_steady_case(underr=0.8)) as g:
+        g.run(40)
+        g.adjoint((), maxit=2, tol=1e-3)
+        for _ in range(3):
+            v, w = rng.standard_normal((2,) + q.shape)
+            lhs = np.vdot(g.adjoint_apply(3, v), w)
+            rhs = np.vdot(v, g.adjoint_apply(2, w))
+            assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
 STEADY = dict(reynum=1000.0, dtau=10.0, underr=1.0, iturb=0, line_order=SMAC_ORDER_ZEBRA)
 
 
