@@ -148,7 +148,7 @@ def converged_run(cfg: int, maxit: int, drop: float, adjoint: bool) -> dict:
             if name == "fp64" and adjoint:
                 jl, ju = [(r[1], r[2]) for r in case.resolved_regions() if r[0] == 0][0]
                 modes = grid.bump_modes(case.x, case.y, jl, ju)
-                dcl, dcd, ast = s.adjoint(modes, maxit=4000, tol=1e-10)
+                dcl, dcd, ast = s.adjoint(modes, maxit=8000, tol=1e-10)
                 out["adjoint"] = {
                     "seconds": round(ast["seconds"], 3), "gmres_iters": ast["iters"],
                     "resid": ast["resid"],

@@ -30,7 +30,8 @@ typedef enum smac_status {
     SMAC_ERR_NONFINITE = 7,    /* residual or update produced Inf/NaN                     */
     SMAC_ERR_HIP = 8,          /* a HIP runtime call failed                               */
     SMAC_ERR_NOGPU = 9,        /* no gfx950 device visible                                */
-    SMAC_ERR_STALL = 10        /* an LU-SGS band hand-over wait timed out (bounded spin)  */
+    SMAC_ERR_STALL = 10,       /* an LU-SGS band hand-over wait timed out (bounded spin)  */
+    SMAC_ERR_NOCONV = 11       /* smac_adjoint: GMRES stopped at maxit above 10 x tol     */
 } smac_status;
 
 /* Solver settings: the reference's namelist $input (ref/initia.f:85-89) plus the values its
@@ -158,7 +159,9 @@ smac_status smac_timings(smac_handle *h, double out[7]);
  * steps of operand prefetch in the 3x3 / scalar LU-SGS sweeps), "lusgs_ring" (0 or 1, default 1:
  * the 3x3 sweeps stream their operands through an LDS ring fed by loader wavefronts),
  * "lr_lines" (1..64, default 64: lines per workgroup of the line-relaxation
- * factorisations).  All but the first two are tuning knobs; results do not depend on them.
+ * factorisations), "adj_restart" (4..400, default 100: GMRES restart length of
+ * smac_adjoint).  All but the first two are tuning knobs; results do not depend on them
+ * beyond the solver tolerances.
  * No counterpart in the reference. */
 smac_status smac_set_option(smac_handle *h, const char *name, int value);
 
@@ -168,14 +171,19 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value);
  * s = 0 on every row but the wake-cut targets, the wake-cut relation
  * q(jt,1) = (q(jt,2) + q(jb,2)) / 2 there (what the injection ref/bc.f:571-591 enforces), and
  * clift = CL (the far field reads the CL of the pass before).  The adjoint of that system is
- * solved once for CL and once for CD with GMRES, right-preconditioned by the transpose of the
- * iteration's own LU-SGS update; the mean flow does not read the SA variables, so the SA part
- * of the adjoint vanishes.
+ * solved once for CL and once for CD with restarted GMRES (option "adj_restart", default
+ * 100), right-preconditioned by the transpose of the iteration's own update: the LU-SGS solve
+ * for impsch = 2, the zebra line-relaxation sweeps for impsch = 1 (whatever the line order),
+ * with a Sherman-Morrison correction for the rank-one lift coupling of the far field.  The
+ * mean flow does not read the SA variables, so the SA part of the adjoint vanishes.
  *  - dcl[0], dcd[0]: d/d(alpha), alpha in degrees;
  *  - dcl[1+i], dcd[1+i]: along shape direction i, given as grid displacements dx, dy
  *    (nshape x jmax*kmax each, Fortran order; the metrics follow them as the reference forms
  *    them); nshape may be 0.
- * maxit bounds the GMRES iterations of each solve, tol is the relative residual to reach. */
+ * maxit bounds the GMRES iterations of each solve, tol is the relative residual to reach.
+ * A solve that stops at maxit with a residual above 10 x tol makes the call return
+ * SMAC_ERR_NOCONV after filling dcl, dcd and st (the gradients are then not reliable: a short
+ * restart can stall on the lift mode). */
 typedef struct smac_adjoint_stats {
     int iters[2];      /* GMRES iterations of the CL and the CD adjoint                        */
     double resid[2];   /* their final residuals relative to the right-hand side             */

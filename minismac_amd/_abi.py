@@ -7,7 +7,7 @@ SMAC_OK = 0
 STATUS_NAMES = {
     0: "SMAC_OK", 1: "SMAC_ERR_ARG", 2: "SMAC_ERR_IMPSCH", 3: "SMAC_ERR_IFLXO",
     4: "SMAC_ERR_NEQS", 5: "SMAC_ERR_SIZE", 6: "SMAC_ERR_BC", 7: "SMAC_ERR_NONFINITE",
-    8: "SMAC_ERR_HIP", 9: "SMAC_ERR_NOGPU", 10: "SMAC_ERR_STALL",
+    8: "SMAC_ERR_HIP", 9: "SMAC_ERR_NOGPU", 10: "SMAC_ERR_STALL", 11: "SMAC_ERR_NOCONV",
 }
 SMAC_FP64, SMAC_FP32_LHS = 0, 1
 SMAC_ORDER_REFERENCE, SMAC_ORDER_ZEBRA = 0, 1

@@ -34,8 +34,14 @@
 //    the metric tangent of a shape direction (k_adj_metric_tangent, as the metric is formed:
 //    second-order differences, one-sided at the edges), the residual tangent at fixed q
 //    (k_adj_resid_tangent) and the force tangent (k_adj_force_tangent).
-//  - GMRES(m) with right preconditioning P^T, classical Gram-Schmidt applied twice; its
-//    inner products are one multi-vector kernel per pass.
+//  - GMRES(m) with right preconditioning, classical Gram-Schmidt applied twice; its inner
+//    products are one multi-vector kernel per pass.  The adjoint operator is
+//    Op = K - g f^T with K = -F_q^T, g = CL_q^T and f = F_c (the far field's lift coupling);
+//    P^T approximates K^-1 only, and GMRES with P^T alone stagnates on the rank-one part
+//    (restarted, it loses that direction).  So the preconditioner is P^T with the
+//    Sherman-Morrison correction for the rank-one term:
+//        Pc v = P^T v + u (f . P^T v) / (1 - f . u),     u = P^T g  (formed once),
+//    one extra dot product per application.
 #include "smac_dev.h"
 #include "smac_kernels.h"
 
@@ -873,6 +879,16 @@ void adj_build(AdjCtx &c)
                            c.vnu, c.alpha, c.chord, c.walls, c.met, c.dj, c.q, c.g[0], c.g[1]);
     hipLaunchKernelGGL(k_adj_fc, dim3(nblk(N)), dim3(256), 0, c.sm, d, c.alpha, c.rtype, c.x, c.y,
                        c.fc, c.fa);
+    // the rank-one correction of the preconditioner: u = P^T g and 1 - f . u (skipped when
+    // that is too close to zero to divide by)
+    adj_apply_prec_t(c, c.g[0], c.smu);
+    mdot(c, c.fc, 1, c.smu, c.red);
+    double fu = 0.0;
+    (void)hipMemcpyAsync(&fu, c.red, sizeof fu, hipMemcpyDeviceToHost, c.sm);
+    (void)hipStreamSynchronize(c.sm);
+    const double beta = 1.0 - fu;
+    c.sm_on = std::fabs(beta) > 1e-8 ? 1 : 0;
+    c.sm_binv = c.sm_on ? 1.0 / beta : 0.0;
 }
 
 // out = F_q^T in
@@ -941,6 +957,28 @@ void adj_apply_prec(AdjCtx &c, const double *in, double *out)
                            c.pairs, out, in, out);
 }
 
+// out += u * red[0] * binv
+__global__ void __launch_bounds__(256)
+k_adj_sm_add(long len, const double *__restrict__ red, double binv, const double *__restrict__ u,
+             double *__restrict__ out)
+{
+    const double a = red[0] * binv;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len;
+         e += (long)gridDim.x * blockDim.x)
+        out[e] += a * u[e];
+}
+
+// the preconditioner GMRES uses: P^T with the rank-one correction (see the file header)
+static void adj_apply_prec_gm(AdjCtx &c, const double *in, double *out)
+{
+    adj_apply_prec_t(c, in, out);
+    if (!c.sm_on) return;
+    const long len = 3 * c.d.N;
+    mdot(c, c.fc, 1, out, c.red);
+    hipLaunchKernelGGL(k_adj_sm_add, dim3(nblk(len)), dim3(256), 0, c.sm, len,
+                       (const double *)c.red, c.sm_binv, (const double *)c.smu, out);
+}
+
 // ||v||_2 on the host
 static double norm2(AdjCtx &c, const double *v)
 {
@@ -982,7 +1020,7 @@ int adj_gmres(AdjCtx &c, const double *b, double *x, int maxit, double tol, doub
         int jl = -1;
         for (int j = 0; j < m && it < maxit; ++j) {
             double *vj = c.V + (long)j * len, *vn = c.V + (long)(j + 1) * len;
-            adj_apply_prec_t(c, vj, c.z);
+            adj_apply_prec_gm(c, vj, c.z);
             adj_apply_op(c, c.z, vn);
             // classical Gram-Schmidt, twice
             mdot(c, c.V, j + 1, vn, c.red);
@@ -1018,7 +1056,7 @@ int adj_gmres(AdjCtx &c, const double *b, double *x, int maxit, double tol, doub
             if (std::fabs(gv[j + 1]) <= tol * bn || hn == 0.0) break;
         }
         if (jl < 0) break;
-        // y = H^-1 g (upper triangular), u = sum y_i V_i, x += P^T u
+        // y = H^-1 g (upper triangular), u = sum y_i V_i, x += Pc u
         for (int i = jl; i >= 0; --i) {
             double t = gv[i];
             for (int k = i + 1; k <= jl; ++k) t -= H[i * m + k] * y[k];
@@ -1027,7 +1065,7 @@ int adj_gmres(AdjCtx &c, const double *b, double *x, int maxit, double tol, doub
         (void)hipMemcpyAsync(c.red, y.data(), sizeof(double) * (jl + 1), hipMemcpyHostToDevice, c.sm);
         hipLaunchKernelGGL(k_adj_maxpy, dim3(nb), dim3(256), 0, c.sm, len, jl + 1, 0.0, 1.0,
                            (const double *)c.V, (const double *)c.red, c.t1b);
-        adj_apply_prec_t(c, c.t1b, c.z);
+        adj_apply_prec_gm(c, c.t1b, c.z);
         hipLaunchKernelGGL(k_adj_axpby, dim3(nb), dim3(256), 0, c.sm, len, 1.0, c.z, 1.0, x, 0.0,
                            (const double *)nullptr);
         // true residual r = b - Op x

@@ -73,6 +73,7 @@ struct smac_handle {
     std::vector<BcRegion> regs;
     int nt = 0;
     double rhs0 = 0.0;            // rhsnorm of pass nt = 1 when this handle made it (else 0)
+    int adj_restart = 100;        // GMRES restart length of smac_adjoint
     bool first_sa = true;
     bool keep_lhs = false;
     bool timing = false;
@@ -438,6 +439,11 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
         h->lr_lines = value;
         return SMAC_OK;
     }
+    if (!std::strcmp(name, "adj_restart")) {
+        if (value < 4 || value > 400) return fail(SMAC_ERR_ARG, "adj_restart must be in [4, 400]");
+        h->adj_restart = value;
+        return SMAC_OK;
+    }
     if (!std::strcmp(name, "lusgs_rows")) {
         if (value < 1 || value > 64) return fail(SMAC_ERR_ARG, "lusgs_rows must be in [1, 64]");
         h->lusgs_rows = value;
@@ -756,7 +762,7 @@ static smac_status adj_prepare(smac_handle *h, int m)
         c.rtype = rtd;
         c.inv = invd;
         double **v3[] = {&c.lam[0], &c.lam[1], &c.g[0], &c.g[1], &c.fc, &c.fa, &c.w, &c.z,
-                         &c.t1, &c.t2, &c.t1b, &c.fb};
+                         &c.t1, &c.t2, &c.t1b, &c.fb, &c.smu};
         if ((rc = dalloc(h, &c.jac, 81 * N))) return rc;
         for (int b = 0; b < 5; ++b)
             if ((rc = dalloc(h, &c.tb[b], 9 * N))) return rc;
@@ -822,7 +828,7 @@ smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const dou
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream_sa));
-    const int m = 40;
+    const int m = h->adj_restart;
     smac_status rc = adj_prepare(h, m);
     if (rc) return rc;
     const Dims &d = h->d;
@@ -856,6 +862,11 @@ smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const dou
     }
     if (!(std::isfinite(dcl[0]) && std::isfinite(dcd[0])))
         return fail(SMAC_ERR_NONFINITE, "adjoint gradient is not finite");
+    for (int o = 0; o < 2; ++o)
+        if (its[o] >= maxit && rel[o] > 10.0 * tol)
+            return fail(SMAC_ERR_NOCONV, std::string(o ? "CD" : "CL") + " adjoint: GMRES stopped at " +
+                                             std::to_string(maxit) + " iterations, residual " +
+                                             std::to_string(rel[o]));
     return SMAC_OK;
 }
 

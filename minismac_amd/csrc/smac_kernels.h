@@ -179,6 +179,8 @@ struct AdjCtx {
     LrScan lr_f, lr_t;           // factored lines of M and of M^T (lr = 1)
     LrTWork lrv;
     double *jac, *tb[5], *lam[2], *g[2], *fc, *fa, *V, *w, *z, *t1, *t2, *t1b;
+    double *smu, sm_binv;        // the preconditioner's rank-one correction (see adj_build)
+    int sm_on;
     double *dmet, *ddj, *dxy, *fb, *fpart;
     double *part, *red, *red2;
 };

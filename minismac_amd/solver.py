@@ -157,10 +157,11 @@ class Solver:
         _check(lib().smac_get_array(self.h, name.encode(), dptr(out)), "smac_get_array")
         return out
 
-    def adjoint(self, shape_dirs=(), maxit: int = 2000, tol: float = 1e-12):
+    def adjoint(self, shape_dirs=(), maxit: int = 2000, tol: float = 1e-12, check: bool = True):
         """Gradients of CL and CD at the current (steady) state (smac_adjoint): returns
         (dcl, dcd, stats), dcl[0]/dcd[0] per degree of alpha, then one entry per shape
-        direction, each a (dx, dy) pair of (jmax, kmax) grid displacements."""
+        direction, each a (dx, dy) pair of (jmax, kmax) grid displacements.  check=False
+        returns unconverged solves too (SMAC_ERR_NOCONV), e.g. to build the operators."""
         n = len(shape_dirs)
         npts = self.jmax * self.kmax
         dx = np.zeros((max(n, 1), npts))
@@ -171,8 +172,10 @@ class Solver:
         dcl = np.zeros(1 + n)
         dcd = np.zeros(1 + n)
         st = SmacAdjointStats()
-        _check(lib().smac_adjoint(self.h, n, dptr(dx.ravel()), dptr(dy.ravel()), maxit, tol,
-                                  dptr(dcl), dptr(dcd), C.byref(st)), "smac_adjoint")
+        rc = lib().smac_adjoint(self.h, n, dptr(dx.ravel()), dptr(dy.ravel()), maxit, tol,
+                                dptr(dcl), dptr(dcd), C.byref(st))
+        if not (rc == 11 and not check):
+            _check(rc, "smac_adjoint")
         return dcl, dcd, st.as_dict()
 
     def adjoint_apply(self, op: int, v: np.ndarray) -> np.ndarray:
