@@ -45,9 +45,13 @@ BAND_FWD_TRAFFIC = {"fp64": (69081.0 + 8225.0) * 1024 / 227745,
                     "fp32": (28733.0 + 4207.0) * 1024 / 227745}
 TRAFFIC_SOURCE = "fixed figure from profiles/lusgs_band_945x241_counters.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
 # Algorithmic bytes per grid point of the line-relaxation solve family (two xi and two eta
-# sweeps with their factorisations): 360 + 2*432 per direction in FP64, 180 + 2*228 with the
-# blocks in FP32
-LR_SOLVE_BYTES = {"fp64": 2448, "fp32": 1272}
+# sweeps with their factorisations).  The reference-order chain factors in place and reads
+# the blocks in its sweeps: 360 + 2*432 per direction.  The scan forms factor into 45
+# line-major operand planes (45 read + 45 written per direction), and each sweep reads those
+# 45 planes, s, dq and the neighbouring lines' dq and writes dq (45 LT + 13 FP64 words), plus
+# the eta-major transposes of s and dq (4 x 3 planes read and written): 2*720 + 4*480 + 192 in
+# FP64 and 2*360 + 4*300 + 192 with the blocks in FP32.
+LR_SOLVE_BYTES = {"chain": 2448, "fp64": 3552, "fp32": 2112}
 
 
 def cpu_baseline(case, iters: int) -> dict:
@@ -87,8 +91,9 @@ def kernel_times(case, steps: int, warmup: int) -> dict:
         return s.timings()
 
 
-def line_relaxation_run(cfg: int, steps: int) -> dict:
-    """The line-relaxation pass at the workload size in its three forms."""
+def line_relaxation_run(cfg: int, steps: int, only=None) -> dict:
+    """The line-relaxation pass at the workload size in its three forms (or those named in
+    `only`)."""
     from minismac_amd._abi import SMAC_FP32_LHS, SMAC_ORDER_ZEBRA
     from minismac_amd.case import config_case
     from minismac_amd.solver import Solver
@@ -99,6 +104,8 @@ def line_relaxation_run(cfg: int, steps: int) -> dict:
              ("zebra_fp32", {"line_order": SMAC_ORDER_ZEBRA, "precision": SMAC_FP32_LHS}, {},
               steps)]
     for name, kw, opts, n in forms:
+        if only and name not in only:
+            continue
         case = config_case(cfg, impsch=1, **kw)
         with Solver(case, options=opts, timing=True) as s:
             s.run(1)
@@ -106,7 +113,7 @@ def line_relaxation_run(cfg: int, steps: int) -> dict:
             st, done = s.run(n)
             wall = time.perf_counter() - t0
             tm = s.timings()
-        prec = "fp32" if "fp32" in name else "fp64"
+        prec = "chain" if name == "chain_fp64" else ("fp32" if "fp32" in name else "fp64")
         out[name] = {"ms_per_pass": round(wall / done * 1e3, 4),
                      "solve_ms": round(tm["solve"], 4), "sa_ms": round(tm["sa"], 4),
                      "mcell_iters_s": round(case.npts * done / wall / 1e6, 4),
@@ -234,6 +241,9 @@ def main() -> int:
     ap.add_argument("--batch-cases", type=int, default=64,
                     help="cases in the batched measurement (0 skips it)")
     ap.add_argument("--no-lr", action="store_true", help="skip the line-relaxation forms")
+    ap.add_argument("--lr-forms", default="",
+                    help="comma list of the line-relaxation forms to time (default: all of "
+                         "chain_fp64, scan_fp64, scan_fp32, zebra_fp64, zebra_fp32)")
     ap.add_argument("--no-converge", action="store_true",
                     help="skip the converged FP64 / FP32-LHS comparison and the adjoint")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradients")
@@ -300,7 +310,7 @@ def main() -> int:
             "alg_bytes_per_launch": alg, "avg_ms_per_launch": round(t, 5),
         }
     else:
-        from_bytes = 2448 * case.npts   # the line-relaxation solve family, FP64
+        from_bytes = LR_SOLVE_BYTES["chain"] * case.npts   # the solve family, reference order
         achieved = from_bytes / (fam["solve"] * 1e-3) / 1e9
         line["roofline"] = {
             "bound": "hbm", "kernel": "line relaxation (factor + sweeps, reference order)",
@@ -324,7 +334,8 @@ def main() -> int:
         }
     if not args.no_lr:
         print("bench: line relaxation forms", file=sys.stderr, flush=True)
-        line["line_relaxation"] = line_relaxation_run(args.config, args.steps // 4 or 1)
+        line["line_relaxation"] = line_relaxation_run(
+            args.config, args.steps // 4 or 1, [f for f in args.lr_forms.split(",") if f] or None)
     if not args.no_converge:
         print("bench: converged FP64 / FP32-LHS and adjoint", file=sys.stderr, flush=True)
         line["converged"] = converged_run(args.config, args.converge_maxit, 1e-10,

@@ -754,52 +754,100 @@ __global__ void k_adj_force_tangent(Dims d, int ivis, double vnu, Walls wl,
 // vector kernels of the GMRES driver
 
 // partial dot products of w with each of the nv vectors V[i] (stride len), per block
+// nv dot products V_i . w: each pass over the elements takes MDG vectors at once (w read once
+// per group); wavefront shuffles and one LDS step reduce each block's sums to per-block
+// partials, which k_adj_msum adds in a fixed order (deterministic).  A single product takes
+// the plain one-vector form.
+constexpr int MDG = 8;
+
+SMAC_DEV double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 __global__ void __launch_bounds__(256)
 k_adj_mdot(long len, int nv, const double *__restrict__ V, const double *__restrict__ w,
+           double *__restrict__ part, const int *__restrict__ gate)
+{
+    if (gate && *gate == 0) return;
+    __shared__ double sh[4][MDG];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nb = gridDim.x;
+    for (int i0 = 0; i0 < nv; i0 += MDG) {
+        const int g = nv - i0 < MDG ? nv - i0 : MDG;
+        double acc[MDG];
+#pragma unroll
+        for (int k = 0; k < MDG; ++k) acc[k] = 0.0;
+        const double *v0 = V + (long)i0 * len;
+        if (g == MDG) {
+            for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len;
+                 e += (long)gridDim.x * blockDim.x) {
+                const double we = w[e];
+#pragma unroll
+                for (int k = 0; k < MDG; ++k) acc[k] += v0[(long)k * len + e] * we;
+            }
+        } else {
+            for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len;
+                 e += (long)gridDim.x * blockDim.x) {
+                const double we = w[e];
+#pragma unroll
+                for (int k = 0; k < MDG; ++k)
+                    if (k < g) acc[k] += v0[(long)k * len + e] * we;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MDG; ++k) {
+            const double t = wave_sum(acc[k]);
+            if (lane == 0) sh[wv][k] = t;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < g)
+            part[(long)(i0 + threadIdx.x) * nb + blockIdx.x] =
+                sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_dot1(long len, const double *__restrict__ v, const double *__restrict__ w,
            double *__restrict__ part)
 {
-    __shared__ double sh[256];
-    for (int i = 0; i < nv; ++i) {
-        double acc = 0.0;
-        const double *vi = V + (long)i * len;
-        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len;
-             e += (long)gridDim.x * blockDim.x)
-            acc += vi[e] * w[e];
-        sh[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(long)i * gridDim.x + blockIdx.x] = sh[0];
-        __syncthreads();
-    }
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double acc = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len;
+         e += (long)gridDim.x * blockDim.x)
+        acc += v[e] * w[e];
+    acc = wave_sum(acc);
+    if (lane == 0) sh[wv] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// sums the per-block partials of nv dot products (one block)
-__global__ void k_adj_msum(int nv, int nblk, const double *__restrict__ part,
-                           double *__restrict__ out)
+// out[i] = the sum of product i's nblk partials, one wavefront per product
+__global__ void __launch_bounds__(256)
+k_adj_msum(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out,
+           const int *__restrict__ gate)
 {
-    __shared__ double sh[256];
-    for (int i = 0; i < nv; ++i) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += part[(long)i * nblk + b];
-        sh[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[i] = sh[0];
-        __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (gate && *gate == 0) {
+        for (int i = threadIdx.x; i < nv; i += blockDim.x) out[i] = 0.0;
+        return;
+    }
+    for (int i = wv; i < nv; i += 4) {
+        double a = 0.0;
+        for (int b = lane; b < nblk; b += 64) a += part[(long)i * nblk + b];
+        a = wave_sum(a);
+        if (lane == 0) out[i] = a;
     }
 }
 
-// w = a*w + cs * sum_i c[i] V[i]  (c in device memory)
 __global__ void __launch_bounds__(256)
 k_adj_maxpy(long len, int nv, double a, double cs, const double *__restrict__ V,
-            const double *__restrict__ c, double *__restrict__ w)
+            const double *__restrict__ c, double *__restrict__ w, const int *__restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len;
          e += (long)gridDim.x * blockDim.x) {
         double acc = 0.0;
@@ -845,12 +893,19 @@ static unsigned nblk(long n, long cap = 2048)
 
 constexpr int ADJ_DOT_BLOCKS = 512;
 
-// nv dot products of w with V[0..nv-1] into c.red[0..nv-1] (device)
-static void mdot(AdjCtx &c, const double *V, int nv, const double *w, double *out)
+// nv dot products of w with V[0..nv-1] into out[0..nv-1] (device); with a gate, nothing is
+// computed (and zeros are written) when *gate == 0
+static void mdot(AdjCtx &c, const double *V, int nv, const double *w, double *out,
+                 const int *gate = nullptr)
 {
     const long len = 3 * c.d.N;
-    hipLaunchKernelGGL(k_adj_mdot, dim3(ADJ_DOT_BLOCKS), dim3(256), 0, c.sm, len, nv, V, w, c.part);
-    hipLaunchKernelGGL(k_adj_msum, dim3(1), dim3(256), 0, c.sm, nv, ADJ_DOT_BLOCKS, c.part, out);
+    if (nv == 1)
+        hipLaunchKernelGGL(k_adj_dot1, dim3(ADJ_DOT_BLOCKS), dim3(256), 0, c.sm, len, V, w, c.part);
+    else
+        hipLaunchKernelGGL(k_adj_mdot, dim3(ADJ_DOT_BLOCKS), dim3(256), 0, c.sm, len, nv, V, w,
+                           c.part, gate);
+    hipLaunchKernelGGL(k_adj_msum, dim3(1), dim3(256), 0, c.sm, nv, ADJ_DOT_BLOCKS, c.part, out,
+                       gate);
 }
 
 void adj_build(AdjCtx &c)
@@ -957,6 +1012,17 @@ void adj_apply_prec(AdjCtx &c, const double *in, double *out)
                            c.pairs, out, in, out);
 }
 
+// gate = 1 when the projections red[0..n-1] removed more than half of the square norm
+// red[n] of the vector (then it is projected a second time)
+__global__ void k_adj_dgks(int n, const double *__restrict__ red, int *__restrict__ gate)
+{
+    double s2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) s2 += red[i] * red[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off, 64);
+    if (threadIdx.x == 0) *gate = (red[n] - s2 < 0.5 * red[n]) ? 1 : 0;
+}
+
 // out += u * red[0] * binv
 __global__ void __launch_bounds__(256)
 k_adj_sm_add(long len, const double *__restrict__ red, double binv, const double *__restrict__ u,
@@ -1022,14 +1088,20 @@ int adj_gmres(AdjCtx &c, const double *b, double *x, int maxit, double tol, doub
             double *vj = c.V + (long)j * len, *vn = c.V + (long)(j + 1) * len;
             adj_apply_prec_gm(c, vj, c.z);
             adj_apply_op(c, c.z, vn);
-            // classical Gram-Schmidt, twice
-            mdot(c, c.V, j + 1, vn, c.red);
+            // classical Gram-Schmidt; its products with the basis and vn . vn in one pass (vn
+            // is V_(j+1)), then a second pass only where the first cancelled more than half of
+            // vn's square norm (the DGKS test, decided on the device)
+            mdot(c, c.V, j + 2, vn, c.red);
+            hipLaunchKernelGGL(k_adj_dgks, dim3(1), dim3(64), 0, c.sm, j + 1,
+                               (const double *)c.red, c.gate);
             hipLaunchKernelGGL(k_adj_maxpy, dim3(nb), dim3(256), 0, c.sm, len, j + 1, 1.0, -1.0,
-                               (const double *)c.V, (const double *)c.red, vn);
+                               (const double *)c.V, (const double *)c.red, vn,
+                               (const int *)nullptr);
             (void)hipMemcpyAsync(h.data(), c.red, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c.sm);
-            mdot(c, c.V, j + 1, vn, c.red2);
+            mdot(c, c.V, j + 1, vn, c.red2, c.gate);
             hipLaunchKernelGGL(k_adj_maxpy, dim3(nb), dim3(256), 0, c.sm, len, j + 1, 1.0, -1.0,
-                               (const double *)c.V, (const double *)c.red2, vn);
+                               (const double *)c.V, (const double *)c.red2, vn,
+                               (const int *)c.gate);
             (void)hipMemcpyAsync(h2.data(), c.red2, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c.sm);
             const double hn = norm2(c, vn);   // synchronises
             for (int i = 0; i <= j; ++i) h[i] += h2[i];
@@ -1064,7 +1136,8 @@ int adj_gmres(AdjCtx &c, const double *b, double *x, int maxit, double tol, doub
         }
         (void)hipMemcpyAsync(c.red, y.data(), sizeof(double) * (jl + 1), hipMemcpyHostToDevice, c.sm);
         hipLaunchKernelGGL(k_adj_maxpy, dim3(nb), dim3(256), 0, c.sm, len, jl + 1, 0.0, 1.0,
-                           (const double *)c.V, (const double *)c.red, c.t1b);
+                           (const double *)c.V, (const double *)c.red, c.t1b,
+                           (const int *)nullptr);
         adj_apply_prec_gm(c, c.t1b, c.z);
         hipLaunchKernelGGL(k_adj_axpby, dim3(nb), dim3(256), 0, c.sm, len, 1.0, c.z, 1.0, x, 0.0,
                            (const double *)nullptr);

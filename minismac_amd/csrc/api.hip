@@ -772,7 +772,7 @@ static smac_status adj_prepare(smac_handle *h, int m)
         if ((rc = dalloc(h, &c.dmet, 4 * N)) || (rc = dalloc(h, &c.ddj, N)) ||
             (rc = dalloc(h, &c.dxy, 2 * N)) || (rc = dalloc(h, &c.fpart, 2 * h->d.jm)) ||
             (rc = dalloc(h, &c.part, 512 * (m + 2))) || (rc = dalloc(h, &c.red, m + 2)) ||
-            (rc = dalloc(h, &c.red2, m + 2)))
+            (rc = dalloc(h, &c.red2, m + 2)) || (rc = dalloc(h, &c.gate, 1)))
             return rc;
         c.lr = h->p.impsch == 1;
         if (c.lr) {

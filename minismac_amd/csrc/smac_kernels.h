@@ -183,6 +183,7 @@ struct AdjCtx {
     int sm_on;
     double *dmet, *ddj, *dxy, *fb, *fpart;
     double *part, *red, *red2;
+    int *gate;                   // second Gram-Schmidt pass wanted (device flag)
 };
 void adj_build(AdjCtx &);
 void adj_apply_op(AdjCtx &, const double *in, double *out);
