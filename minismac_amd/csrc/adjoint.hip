@@ -825,22 +825,21 @@ k_adj_dot1(long len, const double *__restrict__ v, const double *__restrict__ w,
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// out[i] = the sum of product i's nblk partials, one wavefront per product
+// out[i] = the sum of product i's nblk partials, one wavefront per product (four per block)
 __global__ void __launch_bounds__(256)
 k_adj_msum(int nv, int nblk, const double *__restrict__ part, double *__restrict__ out,
            const int *__restrict__ gate)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nv) return;
     if (gate && *gate == 0) {
-        for (int i = threadIdx.x; i < nv; i += blockDim.x) out[i] = 0.0;
+        if (lane == 0) out[i] = 0.0;
         return;
     }
-    for (int i = wv; i < nv; i += 4) {
-        double a = 0.0;
-        for (int b = lane; b < nblk; b += 64) a += part[(long)i * nblk + b];
-        a = wave_sum(a);
-        if (lane == 0) out[i] = a;
-    }
+    double a = 0.0;
+    for (int b = lane; b < nblk; b += 64) a += part[(long)i * nblk + b];
+    a = wave_sum(a);
+    if (lane == 0) out[i] = a;
 }
 
 __global__ void __launch_bounds__(256)
@@ -904,8 +903,8 @@ static void mdot(AdjCtx &c, const double *V, int nv, const double *w, double *ou
     else
         hipLaunchKernelGGL(k_adj_mdot, dim3(ADJ_DOT_BLOCKS), dim3(256), 0, c.sm, len, nv, V, w,
                            c.part, gate);
-    hipLaunchKernelGGL(k_adj_msum, dim3(1), dim3(256), 0, c.sm, nv, ADJ_DOT_BLOCKS, c.part, out,
-                       gate);
+    hipLaunchKernelGGL(k_adj_msum, dim3((nv + 3) / 4), dim3(256), 0, c.sm, nv, ADJ_DOT_BLOCKS,
+                       c.part, out, gate);
 }
 
 void adj_build(AdjCtx &c)

@@ -9,15 +9,23 @@ relative at 945x241), so it is held to 1e-12 relative here.  The
 (FP64 arithmetic on the same state) is the FP64 path's bit for bit, and its solve differs
 from the FP64 solve only at FP32 rounding.  The 64-case seeded FP32-LHS batch runs whole:
 every case finite, a sample bit-identical to the same cases stepped alone, and one FP64
-case of the batch bit-identical to the oracle.
+case of the batch bit-identical to the oracle.  The 945x241 workload case itself (Re = 3e6,
+LU-SGS) runs away from the free stream -- the reference's mean flow is laminar -- and the GPU
+follows the oracle's recorded trajectory (tests/golden/lusgs_945x241_divergence.json) bit
+for bit through all 181 finite passes and fails at the same pass.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
 import pyoracle
 from minismac_amd._abi import SMAC_FP32_LHS
 from minismac_amd.case import batch_cases, config_case
-from minismac_amd.solver import Batch, Solver
+from minismac_amd.solver import Batch, SmacError, Solver
+
+from conftest import ROOT
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +99,24 @@ def test_batch_64_cases_fp32lhs():
     finally:
         for s in sol:
             s.close()
+
+
+def test_945x241_re3e6_lusgs_runs_away_exactly_as_the_oracle():
+    with open(os.path.join(ROOT, "tests", "golden", "lusgs_945x241_divergence.json")) as f:
+        gold = json.load(f)
+    case = config_case(1, impsch=2)
+    keys = gold["keys"]
+    print(f"{case.name} LU-SGS: {len(gold['passes'])} passes against the oracle's record",
+          flush=True)
+    with Solver(case) as g:
+        for n, row in enumerate(gold["passes"], 1):
+            st = g.step()
+            for k, v in zip(keys, row):
+                if k == "rhsnorm":
+                    assert st[k] == pytest.approx(v, rel=1e-12, abs=0), (n, k)
+                else:
+                    assert st[k] == v, (n, k, st[k], v)
+        with pytest.raises(SmacError) as ei:
+            g.step()
+        assert ei.value.status == 7          # SMAC_ERR_NONFINITE
+    assert gold["first_nonfinite_pass"] == len(gold["passes"]) + 1
