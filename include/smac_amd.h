@@ -59,8 +59,10 @@ typedef struct smac_params {
 
 enum { SMAC_FP64 = 0, SMAC_FP32_LHS = 1 };
 /* SMAC_ORDER_REFERENCE: lines in the reference's Gauss-Seidel order (per-iteration parity).
- * SMAC_ORDER_ZEBRA: odd lines, then even lines, each colour fully parallel (a different
- * iteration with the same fixed point; judged at convergence). */
+ * SMAC_ORDER_ZEBRA (line relaxation only): lines 1, 3, 5, ... (1-based), then 2, 4, ...,
+ * every line of a colour at once, in each sweep direction; the SA's k lines follow the same
+ * order.  A different iteration with the same fixed point, judged at convergence.  FP32-LHS
+ * and zebra line relaxation use the parallel line solves (option "lr_solve" = 1). */
 enum { SMAC_ORDER_REFERENCE = 0, SMAC_ORDER_ZEBRA = 1 };
 
 /* One BC region of bcmain.dat with indices already resolved to 1..jmax / 1..kmax
