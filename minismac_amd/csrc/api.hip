@@ -921,7 +921,70 @@ struct smac_batch {
     BandGroup sa;                    // SA LU-SGS (neqs 1, FP64) of the LU-SGS cases
     BandGroup mean[2];               // mean-flow LU-SGS: FP64, FP32-LHS
     std::vector<LusgsWs> snap_mean, snap_sa;   // workspaces the tables were built from
+    // the multi-case form of the per-case kernels (every case LU-SGS; passes after the first)
+    bool multi_ok = false;
+    MultiTab<SaDerivArgs> sa_deriv;
+    MultiTab<SaAsmArgs> sa_asm;
+    MultiTab<SaBcArgs> sa_bc;
+    MultiTab<ShearArgs> sa_shear;
+    MultiTab<MidArgs> sa_mid;
+    MultiTab<UnshearArgs> sa_unshear;
+    MultiTab<WakeArgs> sa_wake;
+    MultiTab<SaUpdArgs> sa_upd;
+    MultiTab<SaTurArgs> sa_tur;
+    MultiTab<AsmArgs> asm_[2];       // [0] FP64 cases, [1] FP32-LHS cases
+    MultiTab<BcArgs> bc[2];
+    MultiTab<ResidPArgs> resp;
+    MultiTab<ResidFArgs> resf;
+    MultiTab<ShearArgs> shear[2];
+    MultiTab<MidArgs> mid[2];
+    MultiTab<UnshearArgs> unshear[2];
+    MultiTab<WakeArgs> wake;
+    MultiTab<UpdArgs> upd;
+    MultiTab<ForceArgs> force;
 };
+
+template <typename A>
+static void mt_free(MultiTab<A> &t)
+{
+    if (t.args) (void)hipFree(t.args);
+    if (t.start) (void)hipFree(t.start);
+    t = MultiTab<A>{};
+}
+
+// a device table of the items with a non-empty grid
+template <typename A>
+static smac_status mt_build(MultiTab<A> &t, const std::vector<A> &items)
+{
+    mt_free(t);
+    std::vector<A> keep;
+    for (const A &a : items)
+        if (a.gx > 0 && a.gy > 0) keep.push_back(a);
+    std::vector<int> start(keep.size() + 1, 0);
+    for (size_t i = 0; i < keep.size(); ++i)
+        start[i + 1] = start[i] + (int)(keep[i].gx * keep[i].gy);
+    t.n = (int)keep.size();
+    t.nblk = start.back();
+    if (t.n == 0) return SMAC_OK;
+    if (hipMalloc(&t.args, keep.size() * sizeof(A)) != hipSuccess ||
+        hipMalloc(&t.start, start.size() * sizeof(int)) != hipSuccess)
+        return fail(SMAC_ERR_HIP, "hipMalloc failed (batch kernel tables)");
+    HIP_TRY(hipMemcpy(t.args, keep.data(), keep.size() * sizeof(A), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+    return SMAC_OK;
+}
+
+static void free_multi(smac_batch *b)
+{
+    mt_free(b->sa_deriv); mt_free(b->sa_asm); mt_free(b->sa_bc); mt_free(b->sa_shear);
+    mt_free(b->sa_mid); mt_free(b->sa_unshear); mt_free(b->sa_wake); mt_free(b->sa_upd);
+    mt_free(b->sa_tur); mt_free(b->resp); mt_free(b->resf); mt_free(b->wake); mt_free(b->upd);
+    mt_free(b->force);
+    for (int g = 0; g < 2; ++g) {
+        mt_free(b->asm_[g]); mt_free(b->bc[g]); mt_free(b->shear[g]); mt_free(b->mid[g]);
+        mt_free(b->unshear[g]);
+    }
+}
 
 #define B_TRY(x)                   \
     do {                           \
@@ -945,6 +1008,7 @@ static void free_group(BandGroup &g)
 static void free_batch(smac_batch *b)
 {
     if (!b) return;
+    free_multi(b);
     free_group(b->sa);
     free_group(b->mean[0]);
     free_group(b->mean[1]);
@@ -984,6 +1048,97 @@ static smac_status build_group(BandGroup &g, const std::vector<BandJob> &jobs, i
     return SMAC_OK;
 }
 
+// The multi-case kernel tables: each item's arguments as its single-case launcher forms them
+// for a pass after the first (nt > 1, the SA's wall distance known).
+static smac_status build_multi(smac_batch *b)
+{
+    free_multi(b);
+    b->multi_ok = true;
+    for (smac_handle *h : b->cases) b->multi_ok = b->multi_ok && is_lusgs(h);
+    if (!b->multi_ok) return SMAC_OK;
+    std::vector<SaDerivArgs> sd;
+    std::vector<SaAsmArgs> sas;
+    std::vector<SaBcArgs> sbc;
+    std::vector<ShearArgs> ssh, sh[2];
+    std::vector<MidArgs> smid, mid[2];
+    std::vector<UnshearArgs> sun, un[2];
+    std::vector<WakeArgs> swk, wk;
+    std::vector<SaUpdArgs> sup;
+    std::vector<SaTurArgs> stu;
+    std::vector<AsmArgs> as[2];
+    std::vector<BcArgs> bc[2];
+    std::vector<ResidPArgs> rp;
+    std::vector<ResidFArgs> rf;
+    std::vector<UpdArgs> up;
+    std::vector<ForceArgs> fo;
+    for (smac_handle *h : b->cases) {
+        const Dims &d = h->d;
+        const DevParams &dp = h->dp;
+        if (has_sa(h)) {
+            SaArrays &a = h->sa;
+            sd.push_back(sa_deriv_args(d, h->q, h->met, h->dj, a));
+            sas.push_back(sa_asm_args(d, dp, 0, h->q, h->met, h->dj, a));
+            sa_bc_args(d, (int)h->regs.size(), h->regs.data(), a, sbc);
+            ssh.push_back(shear_args<double>(d, h->lws_sa, a.sf, a.stc, a.sjm, a.sjp, a.skm,
+                                             a.skp, (double *)nullptr));
+            smid.push_back(mid_args<double, 1>(d, h->lws_sa));
+            sun.push_back(unshear_args(d, h->lws_sa, h->lws_sa.dq, a.ds));
+            swk.push_back(wake_args(d, 1, h->wake, a.anut, a.ds, a.sf));
+            sup.push_back(sa_upd_args(d, dp, h->w, a));
+            stu.push_back(sa_tur_args(h->w, h->st));
+        }
+        const int g = h->p.precision == SMAC_FP32_LHS ? 1 : 0;
+        if (g) {
+            float *fb[5] = {(float *)h->blk[0], (float *)h->blk[1], (float *)h->blk[2],
+                            (float *)h->blk[3], (float *)h->blk[4]};
+            as[g].push_back(asm_args<float>(d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2],
+                                            fb[3], fb[4]));
+            bc_args<float>(d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y, h->met,
+                           h->q, h->s, fb[0], fb[1], fb[2], fb[3], fb[4], bc[g]);
+            sh[g].push_back(shear_args<float>(d, h->lws, h->s, fb[0], fb[1], fb[2], fb[3], fb[4],
+                                              (float *)nullptr));
+            mid[g].push_back(mid_args<float, 3>(d, h->lws));
+        } else {
+            double **db = h->blk;
+            as[g].push_back(asm_args<double>(d, dp, h->q, h->met, h->dj, h->s, db[0], db[1],
+                                             db[2], db[3], db[4]));
+            bc_args<double>(d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y,
+                            h->met, h->q, h->s, db[0], db[1], db[2], db[3], db[4], bc[g]);
+            sh[g].push_back(shear_args<double>(d, h->lws, h->s, db[0], db[1], db[2], db[3], db[4],
+                                               (double *)nullptr));
+            mid[g].push_back(mid_args<double, 3>(d, h->lws));
+        }
+        un[g].push_back(unshear_args(d, h->lws, h->lws.dq, h->dq));
+        rp.push_back(residp_args(d, h->s, h->dj, h->w));
+        rf.push_back(residf_args(d, 2, dp.beta, h->s, h->dj, h->w, h->st));   // nt > 1
+        wk.push_back(wake_args(d, 3, h->wake, h->q, h->dq, h->s));
+        up.push_back(upd_args(d, h->q, h->dq, h->w));
+        fo.push_back(force_args(d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->w, h->st));
+    }
+    B_TRY(mt_build(b->sa_deriv, sd));
+    B_TRY(mt_build(b->sa_asm, sas));
+    B_TRY(mt_build(b->sa_bc, sbc));
+    B_TRY(mt_build(b->sa_shear, ssh));
+    B_TRY(mt_build(b->sa_mid, smid));
+    B_TRY(mt_build(b->sa_unshear, sun));
+    B_TRY(mt_build(b->sa_wake, swk));
+    B_TRY(mt_build(b->sa_upd, sup));
+    B_TRY(mt_build(b->sa_tur, stu));
+    for (int g = 0; g < 2; ++g) {
+        B_TRY(mt_build(b->asm_[g], as[g]));
+        B_TRY(mt_build(b->bc[g], bc[g]));
+        B_TRY(mt_build(b->shear[g], sh[g]));
+        B_TRY(mt_build(b->mid[g], mid[g]));
+        B_TRY(mt_build(b->unshear[g], un[g]));
+    }
+    B_TRY(mt_build(b->resp, rp));
+    B_TRY(mt_build(b->resf, rf));
+    B_TRY(mt_build(b->wake, wk));
+    B_TRY(mt_build(b->upd, up));
+    B_TRY(mt_build(b->force, fo));
+    return SMAC_OK;
+}
+
 // (Re)builds the multi-case band tables from the cases' current workspaces.
 static smac_status build_tables(smac_batch *b)
 {
@@ -1019,7 +1174,7 @@ static smac_status build_tables(smac_batch *b)
     // pf 0 selects the LDS-ring sweeps
     B_TRY(build_group(b->mean[0], jm[0], pf_m[0] < 0 ? 0 : (pf_m[0] ? pf_m[0] : 4)));
     B_TRY(build_group(b->mean[1], jm[1], pf_m[1] < 0 ? 0 : (pf_m[1] ? pf_m[1] : 4)));
-    return SMAC_OK;
+    return build_multi(b);
 }
 
 static smac_status check_tables(smac_batch *b)
@@ -1051,10 +1206,73 @@ static smac_status join_lanes(smac_batch *b)
     return SMAC_OK;
 }
 
+// One pass of every case with each kernel of the pass launched once for all cases (k_multi),
+// in the order of enqueue_pass, on the batch stream: every case gets the kernels and
+// operands of its own pass, so it stays bit-identical to being stepped alone.  For batches of
+// LU-SGS cases past their first pass.
+static smac_status enqueue_batch_pass_multi(smac_batch *b, bool time_band)
+{
+    hipStream_t sm = b->stream;
+    for (smac_handle *h : b->cases) h->nt += 1;
+    // SA up to its band sweeps
+    multi_sa_deriv(sm, b->sa_deriv);
+    multi_sa_assemble(sm, b->sa_asm);
+    multi_sa_bc(sm, b->sa_bc);
+    multi_shear<double, 1>(sm, b->sa_shear);
+    BandGroup &gs = b->sa;
+    if (gs.nblk) {
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true);
+        multi_mid<double, 1>(sm, b->sa_mid);
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false);
+    }
+    multi_unshear<double, 1>(sm, b->sa_unshear);
+    multi_wake(sm, b->sa_wake);
+    multi_sa_update(sm, b->sa_upd);
+    multi_sa_turres(sm, b->sa_tur);
+    // mean flow up to its band sweeps
+    multi_assemble<double>(sm, b->asm_[0]);
+    multi_assemble<float>(sm, b->asm_[1]);
+    multi_bc<double>(sm, b->bc[0]);
+    multi_bc<float>(sm, b->bc[1]);
+    multi_resid_partial(sm, b->resp);
+    multi_resid_final(sm, b->resf);
+    multi_shear<double, 3>(sm, b->shear[0]);
+    multi_shear<float, 3>(sm, b->shear[1]);
+    for (int g = 0; g < 2; ++g) {
+        BandGroup &gm = b->mean[g];
+        if (!gm.nblk) continue;
+        const bool rec = time_band && g == b->band_group;
+        if (rec) HIP_TRY(hipEventRecord(b->ev_band[0], sm));
+        if (g == 0)
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+        else
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+        if (rec) HIP_TRY(hipEventRecord(b->ev_band[1], sm));
+        if (g == 0) {
+            multi_mid<double, 3>(sm, b->mid[0]);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+        } else {
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+        }
+    }
+    multi_unshear<double, 3>(sm, b->unshear[0]);
+    multi_unshear<float, 3>(sm, b->unshear[1]);
+    multi_wake(sm, b->wake);
+    multi_update(sm, b->upd);
+    multi_force(sm, b->force);
+    HIP_TRY(hipGetLastError());
+    return SMAC_OK;
+}
+
 // one pass of every case, in the order of enqueue_pass for each case.  Case i's kernels of a
 // phase run on lane i % BATCH_LANES; the band sweeps of all cases on the batch stream.
 static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
 {
+    if (b->multi_ok) {
+        bool later = true;
+        for (smac_handle *h : b->cases) later = later && h->nt >= 1 && !(has_sa(h) && h->first_sa);
+        if (later) return enqueue_batch_pass_multi(b, time_band);
+    }
     hipStream_t sm = b->stream;
     auto lane_of = [&](int i) { return b->lane[i % BATCH_LANES]; };
     const int nc = (int)b->cases.size();

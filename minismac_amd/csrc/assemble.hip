@@ -205,16 +205,13 @@ SMAC_DEV void visc_dir(const Line5 &L, double vnu, float vf[2], float ud[2], flo
 }
 
 template <typename LT>
-__global__ void __launch_bounds__(256) k_assemble(Dims d, DevParams prm,
-                                                  const double *__restrict__ q,
-                                                  const double *__restrict__ met,
-                                                  const double *__restrict__ dj, double *__restrict__ s,
-                                                  LT *__restrict__ btc, LT *__restrict__ bjm,
-                                                  LT *__restrict__ bjp, LT *__restrict__ bkm,
-                                                  LT *__restrict__ bkp)
+SMAC_DEV void b_assemble(const Blk &bk, Dims d, DevParams prm, const double *__restrict__ q,
+                         const double *__restrict__ met, const double *__restrict__ dj,
+                         double *__restrict__ s, LT *__restrict__ btc, LT *__restrict__ bjm,
+                         LT *__restrict__ bjp, LT *__restrict__ bkm, LT *__restrict__ bkp)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
+    const int j = bk.x * blockDim.x + threadIdx.x;
+    const int k = bk.y;
     if (j >= d.jm || k >= d.km) return;
     const long N = d.N, p = P(d, j, k);
     const bool interior = j > 0 && j < d.jm - 1 && k > 0 && k < d.km - 1;
@@ -284,18 +281,19 @@ __global__ void __launch_bounds__(256) k_assemble(Dims d, DevParams prm,
     }
 }
 
+template <typename LT>
+__global__ void __launch_bounds__(256)
+k_assemble(Dims d, DevParams prm, const double *__restrict__ q, const double *__restrict__ met,
+           const double *__restrict__ dj, double *__restrict__ s, LT *__restrict__ btc,
+           LT *__restrict__ bjm, LT *__restrict__ bjp, LT *__restrict__ bkm, LT *__restrict__ bkp)
+{
+    b_assemble<LT>(hw_blk(), d, prm, q, met, dj, s, btc, bjm, bjp, bkm, bkp);
+}
 
 // ---------------------------------------------------------------------------------------
 // BC rows (ref/bc.f): every region of the case in one single-workgroup launch, region after
 // region in file order (a barrier between regions, so a later region's rows win where two
 // regions share a point, as in the reference's sequence of calls)
-
-constexpr int BC_THREADS = 256;
-constexpr int BC_MAX = 16;
-struct BcTable {
-    int n;
-    BcRegion r[BC_MAX];
-};
 
 // bcoutj, extrapolation branch of ibval 31 plus the fixed-pressure row (ref/bc.f:1627-1676)
 template <typename LT>
@@ -370,11 +368,11 @@ SMAC_DEV void bc_cgridk(const Dims &d, int j, int k, double alpha, double beta, 
 }
 
 template <typename LT>
-__global__ void __launch_bounds__(BC_THREADS)
-k_bc(Dims d, BcTable t, double pout, double alpha, double beta, const DevStats *__restrict__ st,
-     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ met,
-     const double *__restrict__ q, double *__restrict__ s, LT *btc, LT *bjm, LT *bjp, LT *bkm,
-     LT *bkp)
+SMAC_DEV void b_bc(const Dims &d, const BcTable &t, double pout, double alpha, double beta,
+                   const DevStats *__restrict__ st, const double *__restrict__ x,
+                   const double *__restrict__ y, const double *__restrict__ met,
+                   const double *__restrict__ q, double *__restrict__ s, LT *btc, LT *bjm,
+                   LT *bjp, LT *bkm, LT *bkp)
 {
     const double clift = st->clift;
     for (int i = 0; i < t.n; ++i) {
@@ -391,6 +389,16 @@ k_bc(Dims d, BcTable t, double pout, double alpha, double beta, const DevStats *
         }
         __syncthreads();
     }
+}
+
+template <typename LT>
+__global__ void __launch_bounds__(BC_THREADS)
+k_bc(Dims d, BcTable t, double pout, double alpha, double beta, const DevStats *__restrict__ st,
+     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ met,
+     const double *__restrict__ q, double *__restrict__ s, LT *btc, LT *bjm, LT *bjp, LT *bkm,
+     LT *bkp)
+{
+    b_bc<LT>(d, t, pout, alpha, beta, st, x, y, met, q, s, btc, bjm, bjp, bkm, bkp);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -411,15 +419,15 @@ SMAC_DEV ArgMax amax_join(ArgMax a, ArgMax b)
 
 constexpr int RED_THREADS = 256;
 
-__global__ void __launch_bounds__(RED_THREADS)
-k_resid_partial(Dims d, const double *__restrict__ s, const double *__restrict__ dj,
-                double *__restrict__ part_v, long *__restrict__ part_i, double *__restrict__ part_s)
+SMAC_DEV void b_resid_partial(const Blk &bk, Dims d, const double *__restrict__ s,
+                              const double *__restrict__ dj, double *__restrict__ part_v,
+                              long *__restrict__ part_i, double *__restrict__ part_s)
 {
     const long N = d.N, n3 = 3 * N;
     ArgMax all = {-1.0, 0x7fffffffffffffffL}, first = {-1.0, 0x7fffffffffffffffL};
     double ss = 0.0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
-         i += (long)gridDim.x * blockDim.x) {
+    for (long i = (long)bk.x * blockDim.x + threadIdx.x; i < n3;
+         i += (long)bk.nx * blockDim.x) {
         long p = i % N;
         double sv = s[i];
         double a = fabs(sv * dj[p]);
@@ -443,16 +451,23 @@ k_resid_partial(Dims d, const double *__restrict__ s, const double *__restrict__
         __syncthreads();
     }
     if (t == 0) {
-        part_v[2 * blockIdx.x] = sv_[0]; part_i[2 * blockIdx.x] = si_[0];
-        part_v[2 * blockIdx.x + 1] = fv_[0]; part_i[2 * blockIdx.x + 1] = fi_[0];
-        part_s[blockIdx.x] = sq_[0];
+        part_v[2 * bk.x] = sv_[0]; part_i[2 * bk.x] = si_[0];
+        part_v[2 * bk.x + 1] = fv_[0]; part_i[2 * bk.x + 1] = fi_[0];
+        part_s[bk.x] = sq_[0];
     }
 }
 
 __global__ void __launch_bounds__(RED_THREADS)
-k_resid_final(Dims d, int nparts, int nt, double beta, const double *__restrict__ s,
-              const double *__restrict__ dj, const double *__restrict__ part_v,
-              const long *__restrict__ part_i, const double *__restrict__ part_s, DevStats *st)
+k_resid_partial(Dims d, const double *__restrict__ s, const double *__restrict__ dj,
+                double *__restrict__ part_v, long *__restrict__ part_i, double *__restrict__ part_s)
+{
+    b_resid_partial(hw_blk(), d, s, dj, part_v, part_i, part_s);
+}
+
+SMAC_DEV void b_resid_final(const Dims &d, int nparts, int nt, double beta,
+                            const double *__restrict__ s, const double *__restrict__ dj,
+                            const double *__restrict__ part_v, const long *__restrict__ part_i,
+                            const double *__restrict__ part_s, DevStats *st)
 {
     const int t = threadIdx.x;
     ArgMax all = {-1.0, 0x7fffffffffffffffL}, first = {-1.0, 0x7fffffffffffffffL};
@@ -496,6 +511,14 @@ k_resid_final(Dims d, int nparts, int nt, double beta, const double *__restrict_
         st->rhsnorm = sqrt(sq_[0]);
         st->nonfinite = bad || !(st->rhsnorm <= 1.0e300);
     }
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+k_resid_final(Dims d, int nparts, int nt, double beta, const double *__restrict__ s,
+              const double *__restrict__ dj, const double *__restrict__ part_v,
+              const long *__restrict__ part_i, const double *__restrict__ part_s, DevStats *st)
+{
+    b_resid_final(d, nparts, nt, beta, s, dj, part_v, part_i, part_s, st);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -542,6 +565,126 @@ void launch_resid(hipStream_t sm, const Dims &d, int nt, double beta, const doub
     hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(RED_THREADS), 0, sm, d, nb, nt, beta, s, dj,
                        w.red_v, w.red_i, w.red_s, st);
 }
+
+// ---- multi-case launches ------------------------------------------------------------------
+
+template <typename LT>
+AsmArgs asm_args(const Dims &d, const DevParams &prm, const double *q, const double *met,
+                 const double *dj, double *s, LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp)
+{
+    return AsmArgs{(unsigned)((d.jm + 63) / 64), (unsigned)d.km, d, prm, q, met, dj, s, btc, bjm,
+                   bjp, bkm, bkp};
+}
+
+template <typename LT>
+void bc_args(const Dims &d, const DevParams &prm, const BcRegion *regs, int nreg,
+             const DevStats *st, const double *x, const double *y, const double *met,
+             const double *q, double *s, LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp,
+             std::vector<BcArgs> &out)
+{
+    BcTable t{};
+    for (int i = 0; i <= nreg; ++i) {
+        if (i == nreg || t.n == BC_MAX) {
+            if (t.n)
+                out.push_back(BcArgs{1u, 1u, d, t, prm.pout, prm.alpha, prm.beta, st, x, y, met,
+                                     q, s, btc, bjm, bjp, bkm, bkp});
+            t.n = 0;
+            if (i == nreg) break;
+        }
+        const int ib = regs[i].ibcval;
+        if (ib != 31 && ib != 0 && ib != 25) continue;
+        t.r[t.n++] = regs[i];
+    }
+}
+
+ResidPArgs residp_args(const Dims &d, const double *s, const double *dj, Scratch &w)
+{
+    return ResidPArgs{(unsigned)w.nred, 1u, d, s, dj, w.red_v, w.red_i, w.red_s};
+}
+
+ResidFArgs residf_args(const Dims &d, int nt, double beta, const double *s, const double *dj,
+                       Scratch &w, DevStats *st)
+{
+    return ResidFArgs{1u, 1u, d, w.nred, nt, beta, s, dj, w.red_v, w.red_i, w.red_s, st};
+}
+
+template <typename LT>
+struct CAssemble {
+    using Args = AsmArgs;
+    static constexpr int THREADS = 64;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_assemble<LT>(b, a.d, a.prm, a.q, a.met, a.dj, a.s, (LT *)a.btc, (LT *)a.bjm,
+                       (LT *)a.bjp, (LT *)a.bkm, (LT *)a.bkp);
+    }
+};
+
+template <typename LT>
+struct CBc {
+    using Args = BcArgs;
+    static constexpr int THREADS = BC_THREADS;
+    static SMAC_DEV void run(const Blk &, const Args &a)
+    {
+        b_bc<LT>(a.d, a.t, a.pout, a.alpha, a.beta, a.st, a.x, a.y, a.met, a.q, a.s, (LT *)a.btc,
+                 (LT *)a.bjm, (LT *)a.bjp, (LT *)a.bkm, (LT *)a.bkp);
+    }
+};
+
+struct CResidP {
+    using Args = ResidPArgs;
+    static constexpr int THREADS = RED_THREADS;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_resid_partial(b, a.d, a.s, a.dj, a.part_v, a.part_i, a.part_s);
+    }
+};
+
+struct CResidF {
+    using Args = ResidFArgs;
+    static constexpr int THREADS = RED_THREADS;
+    static SMAC_DEV void run(const Blk &, const Args &a)
+    {
+        b_resid_final(a.d, a.nparts, a.nt, a.beta, a.s, a.dj, a.part_v, a.part_i, a.part_s,
+                      a.st);
+    }
+};
+
+template <class C>
+static void multi_launch(hipStream_t sm, const MultiTab<typename C::Args> &t)
+{
+    if (t.n > 0 && t.nblk > 0)
+        hipLaunchKernelGGL(k_multi<C>, dim3(t.nblk), dim3(C::THREADS), 0, sm, t.args, t.start, t.n);
+}
+
+template <typename LT> void multi_assemble(hipStream_t sm, const MultiTab<AsmArgs> &t)
+{
+    multi_launch<CAssemble<LT>>(sm, t);
+}
+template <typename LT> void multi_bc(hipStream_t sm, const MultiTab<BcArgs> &t)
+{
+    multi_launch<CBc<LT>>(sm, t);
+}
+void multi_resid_partial(hipStream_t sm, const MultiTab<ResidPArgs> &t) { multi_launch<CResidP>(sm, t); }
+void multi_resid_final(hipStream_t sm, const MultiTab<ResidFArgs> &t) { multi_launch<CResidF>(sm, t); }
+
+template AsmArgs asm_args<double>(const Dims &, const DevParams &, const double *, const double *,
+                                  const double *, double *, double *, double *, double *,
+                                  double *, double *);
+template AsmArgs asm_args<float>(const Dims &, const DevParams &, const double *, const double *,
+                                 const double *, double *, float *, float *, float *, float *,
+                                 float *);
+template void bc_args<double>(const Dims &, const DevParams &, const BcRegion *, int,
+                              const DevStats *, const double *, const double *, const double *,
+                              const double *, double *, double *, double *, double *, double *,
+                              double *, std::vector<BcArgs> &);
+template void bc_args<float>(const Dims &, const DevParams &, const BcRegion *, int,
+                             const DevStats *, const double *, const double *, const double *,
+                             const double *, double *, float *, float *, float *, float *,
+                             float *, std::vector<BcArgs> &);
+template void multi_assemble<double>(hipStream_t, const MultiTab<AsmArgs> &);
+template void multi_assemble<float>(hipStream_t, const MultiTab<AsmArgs> &);
+template void multi_bc<double>(hipStream_t, const MultiTab<BcArgs> &);
+template void multi_bc<float>(hipStream_t, const MultiTab<BcArgs> &);
 
 template void launch_assemble<double>(hipStream_t, const Dims &, const DevParams &,
                                       const double *, const double *, const double *, double *,

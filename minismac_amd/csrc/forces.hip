@@ -11,12 +11,12 @@
 
 namespace smac {
 
-__global__ void k_update(long n3, double *__restrict__ q, const double *__restrict__ dq,
-                         int *__restrict__ flag)
+SMAC_DEV void b_update(const Blk &bk, long n3, double *__restrict__ q,
+                       const double *__restrict__ dq, int *__restrict__ flag)
 {
     int bad = 0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
-         i += (long)gridDim.x * blockDim.x) {
+    for (long i = (long)bk.x * blockDim.x + threadIdx.x; i < n3;
+         i += (long)bk.nx * blockDim.x) {
         double v = q[i] + dq[i];
         q[i] = v;
         bad |= !(fabs(v) <= 1.0e300);
@@ -24,12 +24,18 @@ __global__ void k_update(long n3, double *__restrict__ q, const double *__restri
     if (bad) atomicOr(flag, 1);
 }
 
+__global__ void k_update(long n3, double *__restrict__ q, const double *__restrict__ dq,
+                         int *__restrict__ flag)
+{
+    b_update(hw_blk(), n3, q, dq, flag);
+}
+
 constexpr int FORCE_THREADS = 256;
 
-__global__ void __launch_bounds__(FORCE_THREADS)
-k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const double *__restrict__ y,
-        const double *__restrict__ met, const double *__restrict__ dj, const double *__restrict__ q,
-        int *__restrict__ flag, DevStats *st)
+SMAC_DEV void b_force(const Dims &d, const DevParams &prm, const Walls &wl,
+                      const double *__restrict__ x, const double *__restrict__ y,
+                      const double *__restrict__ met, const double *__restrict__ dj,
+                      const double *__restrict__ q, int *__restrict__ flag, DevStats *st)
 {
     __shared__ double t_fx[FORCE_THREADS], t_fy[FORCE_THREADS], t_m1[FORCE_THREADS],
         t_m2[FORCE_THREADS], t_zero[FORCE_THREADS];
@@ -136,6 +142,14 @@ k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const dou
     }
 }
 
+__global__ void __launch_bounds__(FORCE_THREADS)
+k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const double *__restrict__ y,
+        const double *__restrict__ met, const double *__restrict__ dj, const double *__restrict__ q,
+        int *__restrict__ flag, DevStats *st)
+{
+    b_force(d, prm, wl, x, y, met, dj, q, flag, st);
+}
+
 void launch_update_force(hipStream_t sm, const Dims &d, const DevParams &prm, const Walls &wl,
                          const double *x, const double *y, const double *met, const double *dj,
                          double *q, const double *dq, Scratch &w, DevStats *st)
@@ -145,6 +159,48 @@ void launch_update_force(hipStream_t sm, const Dims &d, const DevParams &prm, co
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, sm, 3 * d.N, q, dq, w.flag);
     hipLaunchKernelGGL(k_force, dim3(1), dim3(FORCE_THREADS), 0, sm, d, prm, wl, x, y, met, dj, q,
                        w.flag, st);
+}
+
+// ---- multi-case launches ------------------------------------------------------------------
+
+UpdArgs upd_args(const Dims &d, double *q, const double *dq, Scratch &w)
+{
+    int nb = (int)((3 * d.N + 255) / 256);
+    nb = nb > 4096 ? 4096 : nb;
+    return UpdArgs{(unsigned)nb, 1u, 3 * d.N, q, dq, w.flag};
+}
+
+ForceArgs force_args(const Dims &d, const DevParams &prm, const Walls &wl, const double *x,
+                     const double *y, const double *met, const double *dj, const double *q,
+                     Scratch &w, DevStats *st)
+{
+    return ForceArgs{1u, 1u, d, prm, wl, x, y, met, dj, q, w.flag, st};
+}
+
+struct CUpdate {
+    using Args = UpdArgs;
+    static constexpr int THREADS = 256;
+    static SMAC_DEV void run(const Blk &b, const Args &a) { b_update(b, a.n3, a.q, a.dq, a.flag); }
+};
+struct CForce {
+    using Args = ForceArgs;
+    static constexpr int THREADS = FORCE_THREADS;
+    static SMAC_DEV void run(const Blk &, const Args &a)
+    {
+        b_force(a.d, a.prm, a.wl, a.x, a.y, a.met, a.dj, a.q, a.flag, a.st);
+    }
+};
+
+void multi_update(hipStream_t sm, const MultiTab<UpdArgs> &t)
+{
+    if (t.n > 0 && t.nblk > 0)
+        hipLaunchKernelGGL(k_multi<CUpdate>, dim3(t.nblk), dim3(256), 0, sm, t.args, t.start, t.n);
+}
+void multi_force(hipStream_t sm, const MultiTab<ForceArgs> &t)
+{
+    if (t.n > 0 && t.nblk > 0)
+        hipLaunchKernelGGL(k_multi<CForce>, dim3(t.nblk), dim3(FORCE_THREADS), 0, sm, t.args,
+                           t.start, t.n);
 }
 
 }  // namespace smac

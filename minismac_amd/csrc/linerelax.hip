@@ -331,11 +331,11 @@ k_sweep_ref(Dims d, int eta, int nsweep, double underr, double *__restrict__ dq,
 
 // wake-cut injection, bcimpds single-rank branch (ref/bc.f:571-591): k = 0 row of the cut.
 // Pairs whose indices fall outside the row are skipped.
-__global__ void k_wake(long N, int jm, int neqs, int npair, int base,
-                       const int *__restrict__ pairs, const double *__restrict__ sq,
-                       double *__restrict__ ds, double *__restrict__ rhs)
+SMAC_DEV void b_wake(const Blk &bk, long N, int jm, int neqs, int npair, int base,
+                     const int *__restrict__ pairs, const double *__restrict__ sq,
+                     double *__restrict__ ds, double *__restrict__ rhs)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bk.x * blockDim.x + threadIdx.x;
     if (i >= npair) return;
     const long jt = pairs[2 * i] - base, jb = pairs[2 * i + 1] - base;
     if (jt < 0 || jt >= jm || jb < 0 || jb >= jm) return;
@@ -347,11 +347,40 @@ __global__ void k_wake(long N, int jm, int neqs, int npair, int base,
     }
 }
 
+__global__ void k_wake(long N, int jm, int neqs, int npair, int base,
+                       const int *__restrict__ pairs, const double *__restrict__ sq,
+                       double *__restrict__ ds, double *__restrict__ rhs)
+{
+    b_wake(hw_blk(), N, jm, neqs, npair, base, pairs, sq, ds, rhs);
+}
+
 __global__ void k_fill(double *__restrict__ a, long n, double v)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x)
         a[i] = v;
+}
+
+WakeArgs wake_args(const Dims &d, int neqs, const WakePairs &w, const double *sq, double *ds,
+                   double *rhs)
+{
+    return WakeArgs{(unsigned)(w.n > 0 ? (w.n + 63) / 64 : 0), 1u, d.N, d.jm, neqs, w.n, w.base,
+                    w.pairs, sq, ds, rhs};
+}
+
+struct CWake {
+    using Args = WakeArgs;
+    static constexpr int THREADS = 64;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_wake(b, a.N, a.jm, a.neqs, a.npair, a.base, a.pairs, a.sq, a.ds, a.rhs);
+    }
+};
+
+void multi_wake(hipStream_t sm, const MultiTab<WakeArgs> &t)
+{
+    if (t.n > 0 && t.nblk > 0)
+        hipLaunchKernelGGL(k_multi<CWake>, dim3(t.nblk), dim3(64), 0, sm, t.args, t.start, t.n);
 }
 
 void launch_wake(hipStream_t sm, const Dims &d, int neqs, const WakePairs &w, const double *sq,

@@ -136,22 +136,22 @@ SMAC_DEV void tile_from_natural(const Dims &d, int j0, int k0, double (*lds)[TK]
 // factored on the way (and written to `fac` in the natural layout when non-null); then
 // bjm, bkm, bjp, bkp and the right-hand side.  Block 0 also resets the edge buffers.
 template <typename LT, int NEQ>
-__global__ void __launch_bounds__(TILE_THREADS)
-k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict__ btc,
-        const LT *__restrict__ bjm, const LT *__restrict__ bkm, const LT *__restrict__ bjp,
-        const LT *__restrict__ bkp, LT *__restrict__ fac)
+SMAC_DEV void b_shear(const Blk &bk, const Dims &d, const LusgsWs &w,
+                      const double *__restrict__ rhs, const LT *__restrict__ btc,
+                      const LT *__restrict__ bjm, const LT *__restrict__ bkm,
+                      const LT *__restrict__ bjp, const LT *__restrict__ bkp, LT *__restrict__ fac)
 {
     constexpr int NB = NEQ * NEQ, PW = pair_width<NEQ, LT>();
     __shared__ double lds[NB][TK][TPAD];
-    const int j0 = blockIdx.x * TJ, k0 = blockIdx.y * TK;
+    const int j0 = bk.x * TJ, k0 = bk.y * TK;
     LT *wl = (LT *)w.lhs;
     const long gstride = (long)pairs_of<NEQ, LT>(NB) * w.ns * PW;   // elements per block group
 
     {
         // hand-over slots of both sweeps: the jm real ones become the sentinel, the padding
         // on either side stays a finite value (the dummy region after them is not touched)
-        const long nthr = (long)gridDim.x * gridDim.y * TILE_THREADS;
-        const long t0 = ((long)blockIdx.y * gridDim.x + blockIdx.x) * TILE_THREADS + threadIdx.x;
+        const long nthr = (long)bk.nx * bk.ny * TILE_THREADS;
+        const long t0 = ((long)bk.y * bk.nx + bk.x) * TILE_THREADS + threadIdx.x;
         const long nreal = 2L * w.nband * w.epitch;
         for (long i = t0; i < nreal; i += nthr) {
             const long pos = (i % w.epitch) / 4;
@@ -233,14 +233,23 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
     tile_to_sheared<LT, NEQ, NEQ>(w, d.jm, d.km, j0, k0, lds, (LT *)w.r);   // the solve's type
 }
 
+template <typename LT, int NEQ>
+__global__ void __launch_bounds__(TILE_THREADS)
+k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict__ btc,
+        const LT *__restrict__ bjm, const LT *__restrict__ bkm, const LT *__restrict__ bjp,
+        const LT *__restrict__ bkp, LT *__restrict__ fac)
+{
+    b_shear<LT, NEQ>(hw_blk(), d, w, rhs, btc, bjm, bkm, bjp, bkp, fac);
+}
+
 // sheared -> natural (FP64) for the NEQ planes of a vector (dq, or the middle-step rhs)
 template <int NEQ, typename T>
-__global__ void __launch_bounds__(TILE_THREADS)
-k_unshear(Dims d, LusgsWs w, const T *__restrict__ src, double *__restrict__ dst)
+SMAC_DEV void b_unshear(const Blk &bk, const Dims &d, const LusgsWs &w, const T *__restrict__ src,
+                        double *__restrict__ dst)
 {
     constexpr int NP = NEQ;
     __shared__ double lds[NP][TK][TPAD];
-    const int j0 = blockIdx.x * TJ, k0 = blockIdx.y * TK;
+    const int j0 = bk.x * TJ, k0 = bk.y * TK;
     for (int e = threadIdx.x; e < (TJ + TK - 1) * TK; e += TILE_THREADS) {
         const int kk = e % TK, hh = e / TK, jj = hh - kk;
         const int j = j0 + jj, k = k0 + kk;
@@ -259,18 +268,25 @@ k_unshear(Dims d, LusgsWs w, const T *__restrict__ src, double *__restrict__ dst
     }
 }
 
+template <int NEQ, typename T>
+__global__ void __launch_bounds__(TILE_THREADS)
+k_unshear(Dims d, LusgsWs w, const T *__restrict__ src, double *__restrict__ dst)
+{
+    b_unshear<NEQ, T>(hw_blk(), d, w, src, dst);
+}
+
 // ------------------------------------------------------------------------------------------
 // middle step: rhs <- D rhs, pointwise (ref/ilusol.f:307-341 for neqs = 3; ref/ilusol.f:548-556
 // for neqs = 1, where D = 1/b)
 
 template <typename LT, int NEQ>
-__global__ void k_mid(Dims d, LusgsWs w)
+SMAC_DEV void b_mid(const Blk &bk, const Dims &d, const LusgsWs &w)
 {
     constexpr int NB = NEQ * NEQ;
     const LT *F = (const LT *)w.lhs;
     LT *R = (LT *)w.r;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < w.ns;
-         i += (long)gridDim.x * blockDim.x) {
+    for (long i = (long)bk.x * blockDim.x + threadIdx.x; i < w.ns;
+         i += (long)bk.nx * blockDim.x) {
         const int h = (int)(i / w.ks) - HP0, k = (int)(i % w.ks), j = h - k;
         if (k >= d.km || j < 0 || j >= d.jm) continue;
         if constexpr (NEQ == 3) {
@@ -297,6 +313,12 @@ __global__ void k_mid(Dims d, LusgsWs w)
             R[i] = R[i] / F[i];
         }
     }
+}
+
+template <typename LT, int NEQ>
+__global__ void k_mid(Dims d, LusgsWs w)
+{
+    b_mid<LT, NEQ>(hw_blk(), d, w);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1105,6 +1127,95 @@ void lusgs_post(hipStream_t sm, const Dims &d, const LusgsWs &w, const WakePairs
                            (const LT *)w.r, rhs);
     launch_wake(sm, d, NEQ, wk, q, dq, rhs);
 }
+
+// ---- multi-case launches ------------------------------------------------------------------
+
+template <typename LT>
+ShearArgs shear_args(const Dims &d, const LusgsWs &w, const double *rhs, const LT *btc,
+                     const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *fac)
+{
+    return ShearArgs{(unsigned)((d.jm + TJ - 1) / TJ), (unsigned)((d.km + TK - 1) / TK), d, w, rhs,
+                     btc, bjm, bkm, bjp, bkp, fac};
+}
+
+template <typename LT, int NEQ>
+MidArgs mid_args(const Dims &d, const LusgsWs &w)
+{
+    if (NEQ == 3 && sizeof(LT) == 4) return MidArgs{0u, 1u, d, w};   // scaled form
+    long nb = (w.ns + 255) / 256;
+    return MidArgs{(unsigned)(nb > 4096 ? 4096 : nb), 1u, d, w};
+}
+
+UnshearArgs unshear_args(const Dims &d, const LusgsWs &w, const void *src, double *dst)
+{
+    return UnshearArgs{(unsigned)((d.jm + TJ - 1) / TJ), (unsigned)((d.km + TK - 1) / TK), d, w,
+                       src, dst};
+}
+
+template <typename LT, int NEQ>
+struct CShear {
+    using Args = ShearArgs;
+    static constexpr int THREADS = TILE_THREADS;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_shear<LT, NEQ>(b, a.d, a.w, a.rhs, (const LT *)a.btc, (const LT *)a.bjm,
+                         (const LT *)a.bkm, (const LT *)a.bjp, (const LT *)a.bkp, (LT *)a.fac);
+    }
+};
+template <typename LT, int NEQ>
+struct CMid {
+    using Args = MidArgs;
+    static constexpr int THREADS = 256;
+    static SMAC_DEV void run(const Blk &b, const Args &a) { b_mid<LT, NEQ>(b, a.d, a.w); }
+};
+template <typename LT, int NEQ>
+struct CUnshear {
+    using Args = UnshearArgs;
+    static constexpr int THREADS = TILE_THREADS;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_unshear<NEQ, LT>(b, a.d, a.w, (const LT *)a.src, a.dst);
+    }
+};
+
+template <class C>
+static void multi_launch(hipStream_t sm, const MultiTab<typename C::Args> &t)
+{
+    if (t.n > 0 && t.nblk > 0)
+        hipLaunchKernelGGL(k_multi<C>, dim3(t.nblk), dim3(C::THREADS), 0, sm, t.args, t.start, t.n);
+}
+
+template <typename LT, int NEQ> void multi_shear(hipStream_t sm, const MultiTab<ShearArgs> &t)
+{
+    multi_launch<CShear<LT, NEQ>>(sm, t);
+}
+template <typename LT, int NEQ> void multi_mid(hipStream_t sm, const MultiTab<MidArgs> &t)
+{
+    multi_launch<CMid<LT, NEQ>>(sm, t);
+}
+template <typename LT, int NEQ> void multi_unshear(hipStream_t sm, const MultiTab<UnshearArgs> &t)
+{
+    multi_launch<CUnshear<LT, NEQ>>(sm, t);
+}
+
+template ShearArgs shear_args<double>(const Dims &, const LusgsWs &, const double *,
+                                      const double *, const double *, const double *,
+                                      const double *, const double *, double *);
+template ShearArgs shear_args<float>(const Dims &, const LusgsWs &, const double *, const float *,
+                                     const float *, const float *, const float *, const float *,
+                                     float *);
+template MidArgs mid_args<double, 1>(const Dims &, const LusgsWs &);
+template MidArgs mid_args<double, 3>(const Dims &, const LusgsWs &);
+template MidArgs mid_args<float, 3>(const Dims &, const LusgsWs &);
+template void multi_shear<double, 1>(hipStream_t, const MultiTab<ShearArgs> &);
+template void multi_shear<double, 3>(hipStream_t, const MultiTab<ShearArgs> &);
+template void multi_shear<float, 3>(hipStream_t, const MultiTab<ShearArgs> &);
+template void multi_mid<double, 1>(hipStream_t, const MultiTab<MidArgs> &);
+template void multi_mid<double, 3>(hipStream_t, const MultiTab<MidArgs> &);
+template void multi_mid<float, 3>(hipStream_t, const MultiTab<MidArgs> &);
+template void multi_unshear<double, 1>(hipStream_t, const MultiTab<UnshearArgs> &);
+template void multi_unshear<double, 3>(hipStream_t, const MultiTab<UnshearArgs> &);
+template void multi_unshear<float, 3>(hipStream_t, const MultiTab<UnshearArgs> &);
 
 template <typename LT, int NEQ>
 static void launch_lusgs(hipStream_t sm, const Dims &d, const LusgsWs &w, const WakePairs &wk,

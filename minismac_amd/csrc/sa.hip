@@ -55,13 +55,13 @@ __global__ void k_sa_first(Dims d, Walls wl, const double *__restrict__ x,
 
 // first derivatives of u, v (vortsa / curvature loops 10-80), the vorticity, and the
 // point-local part of the curvature correction (curvature loop 100)
-__global__ void k_sa_deriv(Dims d, const double *__restrict__ q, const double *__restrict__ met,
-                           const double *__restrict__ dj, double *__restrict__ vort,
-                           double *__restrict__ cu, double *__restrict__ ck,
-                           double *__restrict__ cv, double *__restrict__ fr)
+SMAC_DEV void b_sa_deriv(const Blk &bk, const Dims &d, const double *__restrict__ q,
+                         const double *__restrict__ met, const double *__restrict__ dj,
+                         double *__restrict__ vort, double *__restrict__ cu,
+                         double *__restrict__ ck, double *__restrict__ cv, double *__restrict__ fr)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
+    const int j = bk.x * blockDim.x + threadIdx.x;
+    const int k = bk.y;
     if (j >= d.jm) return;
     const long N = d.N, p = P(d, j, k);
     const double *u = q + N, *v = q + 2 * N;
@@ -106,19 +106,28 @@ __global__ void k_sa_deriv(Dims d, const double *__restrict__ q, const double *_
     fr[p] = f;
 }
 
-// tsafill loops 180-260 + tsaibt at one point (ref/tursa.f:597-917, 1353-1367)
-__global__ void k_sa_assemble(Dims d, DevParams prm, const double *__restrict__ q,
-                              const double *__restrict__ met, const double *__restrict__ dj,
-                              const double *__restrict__ anut, const double *__restrict__ smin,
-                              int use_smin, const double *__restrict__ vort,
-                              const double *__restrict__ cu, const double *__restrict__ ck,
-                              const double *__restrict__ cv, const double *__restrict__ fr,
-                              double *__restrict__ sjm, double *__restrict__ sjp,
-                              double *__restrict__ skm, double *__restrict__ skp,
-                              double *__restrict__ stc, double *__restrict__ sf)
+__global__ void k_sa_deriv(Dims d, const double *__restrict__ q, const double *__restrict__ met,
+                           const double *__restrict__ dj, double *__restrict__ vort,
+                           double *__restrict__ cu, double *__restrict__ ck,
+                           double *__restrict__ cv, double *__restrict__ fr)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
+    b_sa_deriv(hw_blk(), d, q, met, dj, vort, cu, ck, cv, fr);
+}
+
+// tsafill loops 180-260 + tsaibt at one point (ref/tursa.f:597-917, 1353-1367)
+SMAC_DEV void b_sa_assemble(const Blk &bk, const Dims &d, const DevParams &prm,
+                            const double *__restrict__ q, const double *__restrict__ met,
+                            const double *__restrict__ dj, const double *__restrict__ anut,
+                            const double *__restrict__ smin, int use_smin,
+                            const double *__restrict__ vort, const double *__restrict__ cu,
+                            const double *__restrict__ ck, const double *__restrict__ cv,
+                            const double *__restrict__ fr, double *__restrict__ sjm,
+                            double *__restrict__ sjp, double *__restrict__ skm,
+                            double *__restrict__ skp, double *__restrict__ stc,
+                            double *__restrict__ sf)
+{
+    const int j = bk.x * blockDim.x + threadIdx.x;
+    const int k = bk.y;
     if (j >= d.jm) return;
     const long N = d.N, p = P(d, j, k), s = d.jm;
     if (j == 0 || j == d.jm - 1 || k == 0 || k == d.km - 1) {
@@ -266,11 +275,26 @@ __global__ void k_sa_assemble(Dims d, DevParams prm, const double *__restrict__ 
     sf[p] = vsf;
 }
 
-// tsabc: zero-gradient rows of the outflow regions (ref/tursa.f:1197-1254)
-__global__ void k_sa_bc(Dims d, BcRegion r, const double *__restrict__ anut, double *sjm,
-                        double *sjp, double *skm, double *skp, double *stc, double *sf)
+__global__ void k_sa_assemble(Dims d, DevParams prm, const double *__restrict__ q,
+                              const double *__restrict__ met, const double *__restrict__ dj,
+                              const double *__restrict__ anut, const double *__restrict__ smin,
+                              int use_smin, const double *__restrict__ vort,
+                              const double *__restrict__ cu, const double *__restrict__ ck,
+                              const double *__restrict__ cv, const double *__restrict__ fr,
+                              double *__restrict__ sjm, double *__restrict__ sjp,
+                              double *__restrict__ skm, double *__restrict__ skp,
+                              double *__restrict__ stc, double *__restrict__ sf)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    b_sa_assemble(hw_blk(), d, prm, q, met, dj, anut, smin, use_smin, vort, cu, ck, cv, fr, sjm,
+                  sjp, skm, skp, stc, sf);
+}
+
+// tsabc: zero-gradient rows of the outflow regions (ref/tursa.f:1197-1254)
+SMAC_DEV void b_sa_bc(const Blk &bk, const Dims &d, const BcRegion &r,
+                      const double *__restrict__ anut, double *sjm, double *sjp, double *skm,
+                      double *skp, double *stc, double *sf)
+{
+    const int t = bk.x * blockDim.x + threadIdx.x;
     if (r.jbeg == r.jend) {
         const int k = r.kbeg + t, j = r.jbeg;
         if (k > r.kend) return;
@@ -286,6 +310,12 @@ __global__ void k_sa_bc(Dims d, BcRegion r, const double *__restrict__ anut, dou
         if (k == 0) { skp[p] = -1.0; sf[p] = anut[p + d.jm] - anut[p]; }
         else { skm[p] = -1.0; sf[p] = anut[p - d.jm] - anut[p]; }
     }
+}
+
+__global__ void k_sa_bc(Dims d, BcRegion r, const double *__restrict__ anut, double *sjm,
+                        double *sjp, double *skm, double *skp, double *stc, double *sf)
+{
+    b_sa_bc(hw_blk(), d, r, anut, sjm, sjp, skm, skp, stc, sf);
 }
 
 // lreltur in reference order: ntjsp passes of Gauss-Seidel over k lines with a Thomas solve
@@ -346,14 +376,14 @@ k_sa_lreltur(Dims d, int ntjsp, int npair, const int *__restrict__ pairs,
 }
 
 // tsaupd + eddy viscosity (ref/tursa.f:314-347, 287-292)
-__global__ void __launch_bounds__(256)
-k_sa_update(long N, double vnu, double *__restrict__ anut, const double *__restrict__ ds,
-            double *__restrict__ vnut, double *__restrict__ part)
+SMAC_DEV void b_sa_update(const Blk &bk, long N, double vnu, double *__restrict__ anut,
+                          const double *__restrict__ ds, double *__restrict__ vnut,
+                          double *__restrict__ part)
 {
     const double cv1 = F32(7.1);
     double mx = 0.0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (long)gridDim.x * blockDim.x) {
+    for (long i = (long)bk.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)bk.nx * blockDim.x) {
         double a = dmax_(0.0, anut[i] + ds[i]);
         anut[i] = a;
         mx = dmax_(ds[i], mx);
@@ -368,10 +398,17 @@ k_sa_update(long N, double vnu, double *__restrict__ anut, const double *__restr
         if (threadIdx.x < w) sm[threadIdx.x] = dmax_(sm[threadIdx.x + w], sm[threadIdx.x]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+    if (threadIdx.x == 0) part[bk.x] = sm[0];
 }
 
-__global__ void k_sa_turres(int n, const double *__restrict__ part, DevStats *st)
+__global__ void __launch_bounds__(256)
+k_sa_update(long N, double vnu, double *__restrict__ anut, const double *__restrict__ ds,
+            double *__restrict__ vnut, double *__restrict__ part)
+{
+    b_sa_update(hw_blk(), N, vnu, anut, ds, vnut, part);
+}
+
+SMAC_DEV void b_sa_turres(int n, const double *__restrict__ part, DevStats *st)
 {
     __shared__ double sm[256];
     double mx = 0.0;
@@ -383,6 +420,11 @@ __global__ void k_sa_turres(int n, const double *__restrict__ part, DevStats *st
         __syncthreads();
     }
     if (threadIdx.x == 0) st->turres = sm[0];
+}
+
+__global__ void k_sa_turres(int n, const double *__restrict__ part, DevStats *st)
+{
+    b_sa_turres(n, part, st);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -435,5 +477,94 @@ void launch_sa_update(hipStream_t sm, const Dims &d, const DevParams &prm, Scrat
                        a.vnut, w.sa_v);
     hipLaunchKernelGGL(k_sa_turres, dim3(1), dim3(256), 0, sm, w.nred, w.sa_v, st);
 }
+
+// ---- multi-case launches ------------------------------------------------------------------
+
+SaDerivArgs sa_deriv_args(const Dims &d, const double *q, const double *met, const double *dj,
+                          SaArrays &a)
+{
+    return SaDerivArgs{(unsigned)((d.jm + 63) / 64), (unsigned)d.km, d, q, met, dj, a.vort, a.cu,
+                       a.ck, a.cv, a.fr};
+}
+
+SaAsmArgs sa_asm_args(const Dims &d, const DevParams &prm, int first_pass, const double *q,
+                      const double *met, const double *dj, SaArrays &a)
+{
+    return SaAsmArgs{(unsigned)((d.jm + 63) / 64), (unsigned)d.km, d, prm, q, met, dj, a.anut,
+                     a.smin, first_pass, a.vort, a.cu, a.ck, a.cv, a.fr, a.sjm, a.sjp, a.skm,
+                     a.skp, a.stc, a.sf};
+}
+
+void sa_bc_args(const Dims &d, int nreg, const BcRegion *regs, SaArrays &a,
+                std::vector<SaBcArgs> &out)
+{
+    for (int i = 0; i < nreg; ++i) {
+        const BcRegion &r = regs[i];
+        const int iv = r.ibcval;
+        if (!(iv == 30 || iv == 31 || iv == 32 || iv == 33 || iv == 40)) continue;
+        const int n = r.jbeg == r.jend ? r.kend - r.kbeg + 1 : r.jend - r.jbeg + 1;
+        out.push_back(SaBcArgs{(unsigned)((n + 63) / 64), 1u, d, r, a.anut, a.sjm, a.sjp, a.skm,
+                               a.skp, a.stc, a.sf});
+    }
+}
+
+SaUpdArgs sa_upd_args(const Dims &d, const DevParams &prm, Scratch &w, SaArrays &a)
+{
+    return SaUpdArgs{(unsigned)w.nred, 1u, d.N, prm.vnu, a.anut, a.ds, a.vnut, w.sa_v};
+}
+
+SaTurArgs sa_tur_args(Scratch &w, DevStats *st) { return SaTurArgs{1u, 1u, w.nred, w.sa_v, st}; }
+
+struct CSaDeriv {
+    using Args = SaDerivArgs;
+    static constexpr int THREADS = 64;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_sa_deriv(b, a.d, a.q, a.met, a.dj, a.vort, a.cu, a.ck, a.cv, a.fr);
+    }
+};
+struct CSaAsm {
+    using Args = SaAsmArgs;
+    static constexpr int THREADS = 64;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_sa_assemble(b, a.d, a.prm, a.q, a.met, a.dj, a.anut, a.smin, a.use_smin, a.vort, a.cu,
+                      a.ck, a.cv, a.fr, a.sjm, a.sjp, a.skm, a.skp, a.stc, a.sf);
+    }
+};
+struct CSaBc {
+    using Args = SaBcArgs;
+    static constexpr int THREADS = 64;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_sa_bc(b, a.d, a.r, a.anut, a.sjm, a.sjp, a.skm, a.skp, a.stc, a.sf);
+    }
+};
+struct CSaUpd {
+    using Args = SaUpdArgs;
+    static constexpr int THREADS = 256;
+    static SMAC_DEV void run(const Blk &b, const Args &a)
+    {
+        b_sa_update(b, a.N, a.vnu, a.anut, a.ds, a.vnut, a.part);
+    }
+};
+struct CSaTur {
+    using Args = SaTurArgs;
+    static constexpr int THREADS = 256;
+    static SMAC_DEV void run(const Blk &, const Args &a) { b_sa_turres(a.n, a.part, a.st); }
+};
+
+template <class C>
+static void multi_launch(hipStream_t sm, const MultiTab<typename C::Args> &t)
+{
+    if (t.n > 0 && t.nblk > 0)
+        hipLaunchKernelGGL(k_multi<C>, dim3(t.nblk), dim3(C::THREADS), 0, sm, t.args, t.start, t.n);
+}
+
+void multi_sa_deriv(hipStream_t sm, const MultiTab<SaDerivArgs> &t) { multi_launch<CSaDeriv>(sm, t); }
+void multi_sa_assemble(hipStream_t sm, const MultiTab<SaAsmArgs> &t) { multi_launch<CSaAsm>(sm, t); }
+void multi_sa_bc(hipStream_t sm, const MultiTab<SaBcArgs> &t) { multi_launch<CSaBc>(sm, t); }
+void multi_sa_update(hipStream_t sm, const MultiTab<SaUpdArgs> &t) { multi_launch<CSaUpd>(sm, t); }
+void multi_sa_turres(hipStream_t sm, const MultiTab<SaTurArgs> &t) { multi_launch<CSaTur>(sm, t); }
 
 }  // namespace smac

@@ -44,4 +44,32 @@ struct DevParams {
 SMAC_DEV double dmax_(double a, double b) { return a > b ? a : b; }
 SMAC_DEV double dmin_(double a, double b) { return a < b ? a : b; }
 
+// The block index and grid a kernel body works with: the hardware's in a plain launch
+// (hw_blk), an item's own in a multi-case launch (k_multi).
+struct Blk {
+    unsigned x, y, nx, ny;
+};
+SMAC_DEV Blk hw_blk() { return Blk{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}; }
+
+// One launch for many items (cases, or a case's BC regions) of one kernel: item i's own grid
+// is args[i].gx x args[i].gy blocks, and the launch's blocks are those grids one after the
+// other (start[i] = the first block of item i, start[n] = the total).  Each block finds its
+// item and runs the kernel body C::run with the item's arguments and its block index in the
+// item's grid, so every item gets exactly the work (and the arithmetic) of its own launch.
+template <class C>
+__global__ void __launch_bounds__(C::THREADS)
+k_multi(const typename C::Args *__restrict__ args, const int *__restrict__ start, int n)
+{
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const typename C::Args &a = args[lo];
+    const unsigned lb = (unsigned)(b - start[lo]);
+    C::run(Blk{lb % a.gx, lb / a.gx, a.gx, a.gy}, a);
+}
+
 }  // namespace smac

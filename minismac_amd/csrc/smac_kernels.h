@@ -3,6 +3,8 @@
 
 #include "smac_dev.h"
 
+#include <vector>
+
 namespace smac {
 
 struct BcRegion {
@@ -193,5 +195,190 @@ void adj_apply_jt(AdjCtx &, const double *in, double *out);
 int adj_gmres(AdjCtx &, const double *b, double *x, int maxit, double tol, double *rel);
 void adj_gradients(AdjCtx &, int nshape, const double *dx, const double *dy, double *dcl,
                    double *dcd);
+
+// ---- multi-case launches (the batch pass): per-item arguments, one table per kernel ------
+// A table lives on the device: args[n] and start[n + 1] (see k_multi).
+template <typename A>
+struct MultiTab {
+    A *args = nullptr;
+    int *start = nullptr;
+    int n = 0, nblk = 0;
+};
+
+constexpr int BC_THREADS = 256;
+constexpr int BC_MAX = 16;
+struct BcTable {
+    int n;
+    BcRegion r[BC_MAX];
+};
+
+struct AsmArgs {      // k_assemble
+    unsigned gx, gy;
+    Dims d;
+    DevParams prm;
+    const double *q, *met, *dj;
+    double *s;
+    void *btc, *bjm, *bjp, *bkm, *bkp;   // LT
+};
+struct BcArgs {       // k_bc
+    unsigned gx, gy;
+    Dims d;
+    BcTable t;
+    double pout, alpha, beta;
+    const DevStats *st;
+    const double *x, *y, *met, *q;
+    double *s;
+    void *btc, *bjm, *bjp, *bkm, *bkp;   // LT
+};
+struct ResidPArgs {   // k_resid_partial
+    unsigned gx, gy;
+    Dims d;
+    const double *s, *dj;
+    double *part_v;
+    long *part_i;
+    double *part_s;
+};
+struct ResidFArgs {   // k_resid_final
+    unsigned gx, gy;
+    Dims d;
+    int nparts, nt;
+    double beta;
+    const double *s, *dj, *part_v;
+    const long *part_i;
+    const double *part_s;
+    DevStats *st;
+};
+struct SaDerivArgs {  // k_sa_deriv
+    unsigned gx, gy;
+    Dims d;
+    const double *q, *met, *dj;
+    double *vort, *cu, *ck, *cv, *fr;
+};
+struct SaAsmArgs {    // k_sa_assemble
+    unsigned gx, gy;
+    Dims d;
+    DevParams prm;
+    const double *q, *met, *dj, *anut, *smin;
+    int use_smin;
+    const double *vort, *cu, *ck, *cv, *fr;
+    double *sjm, *sjp, *skm, *skp, *stc, *sf;
+};
+struct SaBcArgs {     // k_sa_bc (one item per outflow region)
+    unsigned gx, gy;
+    Dims d;
+    BcRegion r;
+    const double *anut;
+    double *sjm, *sjp, *skm, *skp, *stc, *sf;
+};
+struct SaUpdArgs {    // k_sa_update
+    unsigned gx, gy;
+    long N;
+    double vnu;
+    double *anut;
+    const double *ds;
+    double *vnut, *part;
+};
+struct SaTurArgs {    // k_sa_turres
+    unsigned gx, gy;
+    int n;
+    const double *part;
+    DevStats *st;
+};
+struct UpdArgs {      // k_update
+    unsigned gx, gy;
+    long n3;
+    double *q;
+    const double *dq;
+    int *flag;
+};
+struct ForceArgs {    // k_force
+    unsigned gx, gy;
+    Dims d;
+    DevParams prm;
+    Walls wl;
+    const double *x, *y, *met, *dj, *q;
+    int *flag;
+    DevStats *st;
+};
+struct WakeArgs {     // k_wake
+    unsigned gx, gy;
+    long N;
+    int jm, neqs, npair, base;
+    const int *pairs;
+    const double *sq;
+    double *ds, *rhs;
+};
+struct ShearArgs {    // k_shear
+    unsigned gx, gy;
+    Dims d;
+    LusgsWs w;
+    const double *rhs;
+    const void *btc, *bjm, *bkm, *bjp, *bkp;   // LT
+    void *fac;                                 // LT
+};
+struct MidArgs {      // k_mid
+    unsigned gx, gy;
+    Dims d;
+    LusgsWs w;
+};
+struct UnshearArgs {  // k_unshear
+    unsigned gx, gy;
+    Dims d;
+    LusgsWs w;
+    const void *src;  // the solve's type
+    double *dst;
+};
+
+// the arguments of one case's launches, as the single-case launchers form them
+template <typename LT>
+AsmArgs asm_args(const Dims &, const DevParams &, const double *q, const double *met,
+                 const double *dj, double *s, LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp);
+ResidPArgs residp_args(const Dims &, const double *s, const double *dj, Scratch &);
+ResidFArgs residf_args(const Dims &, int nt, double beta, const double *s, const double *dj,
+                       Scratch &, DevStats *);
+template <typename LT>
+ShearArgs shear_args(const Dims &, const LusgsWs &, const double *rhs, const LT *btc,
+                     const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *fac);
+template <typename LT, int NEQ>
+MidArgs mid_args(const Dims &, const LusgsWs &);   // gx = 0: no launch (scaled FP32 form)
+UnshearArgs unshear_args(const Dims &, const LusgsWs &, const void *src, double *dst);
+WakeArgs wake_args(const Dims &, int neqs, const WakePairs &, const double *sq, double *ds,
+                   double *rhs);          // gx = 0 when there are no pairs
+UpdArgs upd_args(const Dims &, double *q, const double *dq, Scratch &);
+ForceArgs force_args(const Dims &, const DevParams &, const Walls &, const double *x,
+                     const double *y, const double *met, const double *dj, const double *q,
+                     Scratch &, DevStats *);
+SaDerivArgs sa_deriv_args(const Dims &, const double *q, const double *met, const double *dj,
+                          SaArrays &);
+SaAsmArgs sa_asm_args(const Dims &, const DevParams &, int first_pass, const double *q,
+                      const double *met, const double *dj, SaArrays &);
+// the outflow rows of the SA: one item per region (appended to out)
+void sa_bc_args(const Dims &, int nreg, const BcRegion *regs, SaArrays &,
+                std::vector<SaBcArgs> &out);
+SaUpdArgs sa_upd_args(const Dims &, const DevParams &, Scratch &, SaArrays &);
+SaTurArgs sa_tur_args(Scratch &, DevStats *);
+// the BC rows of a case: one item per launch the single-case launcher would make
+template <typename LT>
+void bc_args(const Dims &, const DevParams &, const BcRegion *regs, int nreg, const DevStats *,
+             const double *x, const double *y, const double *met, const double *q, double *s,
+             LT *btc, LT *bjm, LT *bjp, LT *bkm, LT *bkp, std::vector<BcArgs> &out);
+
+// one launch over all items of a table
+template <typename LT> void multi_assemble(hipStream_t, const MultiTab<AsmArgs> &);
+template <typename LT> void multi_bc(hipStream_t, const MultiTab<BcArgs> &);
+void multi_resid_partial(hipStream_t, const MultiTab<ResidPArgs> &);
+void multi_resid_final(hipStream_t, const MultiTab<ResidFArgs> &);
+template <typename LT, int NEQ> void multi_shear(hipStream_t, const MultiTab<ShearArgs> &);
+template <typename LT, int NEQ> void multi_mid(hipStream_t, const MultiTab<MidArgs> &);
+template <typename LT, int NEQ> void multi_unshear(hipStream_t, const MultiTab<UnshearArgs> &);
+void multi_wake(hipStream_t, const MultiTab<WakeArgs> &);
+void multi_update(hipStream_t, const MultiTab<UpdArgs> &);
+void multi_force(hipStream_t, const MultiTab<ForceArgs> &);
+void multi_sa_deriv(hipStream_t, const MultiTab<SaDerivArgs> &);
+void multi_sa_assemble(hipStream_t, const MultiTab<SaAsmArgs> &);
+void multi_sa_bc(hipStream_t, const MultiTab<SaBcArgs> &);
+void multi_sa_update(hipStream_t, const MultiTab<SaUpdArgs> &);
+void multi_sa_turres(hipStream_t, const MultiTab<SaTurArgs> &);
+
 
 }  // namespace smac
