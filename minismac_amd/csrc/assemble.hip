@@ -119,9 +119,11 @@ SMAC_DEV void load_line(const double *__restrict__ q, const double *__restrict__
 SMAC_DEV void convect_dir(const Line5 &L, int i, int n, double beta, double cdis, double dflux[3],
                           double bm[9], double bp[9], double bc[9])
 {
-    // faces i-2 .. i+1 (face f lies between points f and f+1); index s = f - (i-2)
+    // faces i-2 .. i+1 (face f lies between points f and f+1); index s = f - (i-2).  The
+    // blocks are formed as faces i-1 and i appear (same expressions, same order), so only
+    // face i-1's split is held until face i's.
     double dfp[4][3], dfm[4][3];
-    Face fc[2];   // faces i-1 and i
+    double ap0[9], am0[9];   // face i-1
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         int f = i - 2 + s;
@@ -147,11 +149,33 @@ SMAC_DEV void convect_dir(const Line5 &L, int i, int n, double beta, double cdis
             dfp[s][m] = F.ap[m] * dq1 + F.ap[m + 3] * dq2 + F.ap[m + 6] * dq3;
             dfm[s][m] = F.am[m] * dq1 + F.am[m + 3] * dq2 + F.am[m + 6] * dq3;
         }
-        if (s == 1) fc[0] = F;
-        if (s == 2) fc[1] = F;
+        if (s == 1) {
+            double atm[9];
+            point_jac(L.rx[1], L.ry[1], L.u[1], L.v[1], beta, atm);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                ap0[e] = F.ap[e];
+                am0[e] = F.am[e];
+                bm[e] = 0.5 * (-atm[e] - F.ap[e] + F.am[e]);
+            }
+        }
+        if (s == 2) {
+            double atp[9];
+            point_jac(L.rx[3], L.ry[3], L.u[3], L.v[3], beta, atp);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                bp[e] = 0.5 * (atp[e] - F.ap[e] + F.am[e]);
+                bc[e] = 0.5 * (F.ap[e] + ap0[e] - F.am[e] - am0[e]);
+            }
+        }
     }
     const double r6 = (double)(1.0f / 6.0f);   // a default-REAL quotient in the reference
     double ff[2][3];
+    // Point i is interior (1 <= i <= n-2), so for faces f = i-1 and i the neighbour faces
+    // f+1 and f+2 (clamped at n-1) and the points f, f+1 are the local entries sc = 1+w,
+    // sp = 2+w (no clamp reaches them), and face f-1, clamped at 0, is entry w except for
+    // i = 1, w = 0, where it is face 0 itself (entry 1).  Constant indices keep dfp/dfm in
+    // registers.
 #pragma unroll
     for (int w = 0; w < 2; ++w) {
         int f = i - 1 + w;             // face index; its neighbours are faces f-1 (clamped) and f+1
@@ -160,16 +184,20 @@ SMAC_DEV void convect_dir(const Line5 &L, int i, int n, double beta, double cdis
         int fpp = f + 2 < n - 1 ? f + 2 : n - 1;
         float cp = (float)(fpp - fp);
         float cm = (float)(f - fm);
-        int sm = fm - (i - 2), sc = f - (i - 2), sp = fp - (i - 2);
-        int pf = f - (i - 2), pp = fp - (i - 2);   // local point indices f, min(n-1, f+1)
+        constexpr int sc = 1, sp = 2;  // + w below
+        const bool sm1 = w == 0 && i == 1;
+        const int pf = 1 + w, pp = 2 + w;   // local point indices f, f+1
         double fv0[3], fv1[3];
         point_flux(L.p[pf], L.u[pf], L.v[pf], L.rx[pf], L.ry[pf], beta, fv0);
         point_flux(L.p[pp], L.u[pp], L.v[pp], L.rx[pp], L.ry[pp], beta, fv1);
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            double fmn = (dfp[sc][m] + dfm[sc][m] - dfp[sp][m] - dfm[sp][m]) * r6;
-            double fpl = (dfp[sm][m] + dfm[sm][m] - dfp[sc][m] - dfm[sc][m]) * r6;
-            double f3 = (dfp[sm][m] - dfp[sc][m] + dfm[sc][m] - dfm[sp][m]) * r6;
+            const double pc = dfp[sc + w][m], mc = dfm[sc + w][m];
+            const double pn = dfp[sp + w][m], mn = dfm[sp + w][m];
+            const double pm = sm1 ? dfp[1][m] : dfp[w][m], mm = sm1 ? dfm[1][m] : dfm[w][m];
+            double fmn = (pc + mc - pn - mn) * r6;
+            double fpl = (pm + mm - pc - mc) * r6;
+            double f3 = (pm - pc + mc - mn) * r6;
             ff[w][m] = 0.5 * (fv1[m] + fv0[m]) +
                        cdis * ((double)(cp * cm) * f3 + (double)(cm * (1.f - cp)) * fpl +
                                (double)(cp * (1.f - cm)) * fmn);
@@ -177,15 +205,6 @@ SMAC_DEV void convect_dir(const Line5 &L, int i, int n, double beta, double cdis
     }
 #pragma unroll
     for (int m = 0; m < 3; ++m) dflux[m] = ff[1][m] - ff[0][m];
-    double atm[9], atp[9];
-    point_jac(L.rx[1], L.ry[1], L.u[1], L.v[1], beta, atm);
-    point_jac(L.rx[3], L.ry[3], L.u[3], L.v[3], beta, atp);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) {
-        bm[e] = 0.5 * (-atm[e] - fc[0].ap[e] + fc[0].am[e]);
-        bp[e] = 0.5 * (atp[e] - fc[1].ap[e] + fc[1].am[e]);
-        bc[e] = 0.5 * (fc[1].ap[e] + fc[0].ap[e] - fc[1].am[e] - fc[0].am[e]);
-    }
 }
 
 // viscous face coefficients of one direction at interior point i (ref/visc.f:95-113):
@@ -215,26 +234,52 @@ SMAC_DEV void b_assemble(const Blk &bk, Dims d, DevParams prm, const double *__r
     if (j >= d.jm || k >= d.km) return;
     const long N = d.N, p = P(d, j, k);
     const bool interior = j > 0 && j < d.jm - 1 && k > 0 && k < d.km - 1;
-    double vs[3] = {0.0, 0.0, 0.0};
-    double vc[9], vjm[9], vjp[9], vkm[9], vkp[9];
+    if (!interior) {
+        // blanked boundary row: all blocks and s zero, btc = identity (ref/step.f:386-437)
 #pragma unroll
-    for (int e = 0; e < 9; ++e) vc[e] = vjm[e] = vjp[e] = vkm[e] = vkp[e] = 0.0;
-    if (interior) {
-        const double *xix = met, *xiy = met + N, *etx = met + 2 * N, *ety = met + 3 * N;
-        Line5 L;
-        double df[3], bc[9];
-        // xi: assign (ref/conup.f:216-271)
-        load_line(q, xix, xiy, dj, N, (long)d.jm * k, 1, j, d.jm, L);
-        convect_dir(L, j, d.jm, prm.beta, prm.cdis, df, vjm, vjp, bc);
+        for (int m = 0; m < 3; ++m) s[m * N + p] = 0.0;
 #pragma unroll
-        for (int e = 0; e < 9; ++e) vc[e] = bc[e];
+        for (int e = 0; e < 9; ++e) {
+            btc[e * N + p] = (LT)((e == 0 || e == 4 || e == 8) ? 1.0 : 0.0);
+            bjm[e * N + p] = (LT)0.0;
+            bjp[e * N + p] = (LT)0.0;
+            bkm[e * N + p] = (LT)0.0;
+            bkp[e * N + p] = (LT)0.0;
+        }
+        return;
+    }
+    // Each direction's neighbour blocks are final once that direction is done and are stored
+    // then; the centre block and s take their terms in the reference's order: xi convective,
+    // eta convective, xi viscous, eta viscous, pseudo-time diagonal.
+    const double *xix = met, *xiy = met + N, *etx = met + 2 * N, *ety = met + 3 * N;
+    double vs[3], vc[9], df[3];
+    float vfx[2], udx[2], vdx[2];
+    Line5 L;
+    // xi: assign (ref/conup.f:216-271)
+    load_line(q, xix, xiy, dj, N, (long)d.jm * k, 1, j, d.jm, L);
+    {
+        double bm[9], bp[9];
+        convect_dir(L, j, d.jm, prm.beta, prm.cdis, df, bm, bp, vc);
 #pragma unroll
         for (int m = 0; m < 3; ++m) vs[m] = -df[m];
-        float vfx[2], udx[2], vdx[2];
-        if (prm.ivis == 1) visc_dir(L, prm.vnu, vfx, udx, vdx);
-        // eta: add (ref/conup.f:377-457)
-        load_line(q, etx, ety, dj, N, (long)j, d.jm, k, d.km, L);
-        convect_dir(L, k, d.km, prm.beta, prm.cdis, df, vkm, vkp, bc);
+        if (prm.ivis == 1) {
+            visc_dir(L, prm.vnu, vfx, udx, vdx);
+            // xi viscous blocks (ref/visc.f:95-113)
+            const double a = vfx[0], b = vfx[1];
+            bm[4] = bm[4] - a; bm[8] = bm[8] - a;
+            bp[4] = bp[4] - b; bp[8] = bp[8] - b;
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            bjm[e * N + p] = (LT)bm[e];
+            bjp[e * N + p] = (LT)bp[e];
+        }
+    }
+    // eta: add (ref/conup.f:377-457)
+    load_line(q, etx, ety, dj, N, (long)j, d.jm, k, d.km, L);
+    {
+        double bm[9], bp[9], bc[9];
+        convect_dir(L, k, d.km, prm.beta, prm.cdis, df, bm, bp, bc);
 #pragma unroll
         for (int e = 0; e < 9; ++e) vc[e] = vc[e] + bc[e];
 #pragma unroll
@@ -242,43 +287,37 @@ SMAC_DEV void b_assemble(const Blk &bk, Dims d, DevParams prm, const double *__r
         if (prm.ivis == 1) {
             float vfy[2], udy[2], vdy[2];
             visc_dir(L, prm.vnu, vfy, udy, vdy);
-            // xi viscous (ref/visc.f:95-113), then eta (:121-140)
+            // xi viscous terms of the centre and s (ref/visc.f:95-113), then eta (:121-140)
             double a = vfx[0], b = vfx[1];
-            vjm[4] = vjm[4] - a; vjm[8] = vjm[8] - a;
             vc[4] = vc[4] + a + b; vc[8] = vc[8] + a + b;
-            vjp[4] = vjp[4] - b; vjp[8] = vjp[8] - b;
             float t2 = vfx[1] * udx[1] - vfx[0] * udx[0];
             float t3 = vfx[1] * vdx[1] - vfx[0] * vdx[0];
             vs[1] = vs[1] + (double)t2;
             vs[2] = vs[2] + (double)t3;
             a = vfy[0]; b = vfy[1];
-            vkm[4] = vkm[4] - a; vkm[8] = vkm[8] - a;
+            bm[4] = bm[4] - a; bm[8] = bm[8] - a;
             vc[4] = vc[4] + a + b; vc[8] = vc[8] + a + b;
-            vkp[4] = vkp[4] - b; vkp[8] = vkp[8] - b;
+            bp[4] = bp[4] - b; bp[8] = bp[8] - b;
             t2 = vfy[1] * udy[1] - vfy[0] * udy[0];
             t3 = vfy[1] * vdy[1] - vfy[0] * vdy[0];
             vs[1] = vs[1] + (double)t2;
             vs[2] = vs[2] + (double)t3;
         }
-        // pseudo-time diagonal (ref/step.f:419-437), fib = 1
-        double rdjdtau = 1.0 / (dj[p] * prm.dtau) + 1. - 1.0;
-        vc[0] = vc[0] + rdjdtau;
-        vc[4] = vc[4] + rdjdtau;
-        vc[8] = vc[8] + rdjdtau;
-    } else {
-        // blanked boundary row: all blocks and s zero, btc = identity (ref/step.f:386-437)
-        vc[0] = vc[4] = vc[8] = 1.0;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            bkm[e * N + p] = (LT)bm[e];
+            bkp[e * N + p] = (LT)bp[e];
+        }
     }
+    // pseudo-time diagonal (ref/step.f:419-437), fib = 1
+    double rdjdtau = 1.0 / (dj[p] * prm.dtau) + 1. - 1.0;
+    vc[0] = vc[0] + rdjdtau;
+    vc[4] = vc[4] + rdjdtau;
+    vc[8] = vc[8] + rdjdtau;
 #pragma unroll
     for (int m = 0; m < 3; ++m) s[m * N + p] = vs[m];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
-        btc[e * N + p] = (LT)vc[e];
-        bjm[e * N + p] = (LT)vjm[e];
-        bjp[e * N + p] = (LT)vjp[e];
-        bkm[e * N + p] = (LT)vkm[e];
-        bkp[e * N + p] = (LT)vkp[e];
-    }
+    for (int e = 0; e < 9; ++e) btc[e * N + p] = (LT)vc[e];
 }
 
 template <typename LT>
