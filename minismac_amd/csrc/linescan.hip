@@ -162,50 +162,70 @@ SMAC_DEV void mm(const T *x, const T *y, T *out)
 
 constexpr int FW = 64;   // points per chunk = lanes of the wave
 
+// The line is split at m = len/2 and eliminated from both ends at once (a twisted
+// factorisation): wave 0 runs the recurrence forward over points 0..m-1,
+//     T_i = b_i - a_i C_(i-1),   C_i = T_i^-1 c_i,      M_i = -T_i^-1 a_i,
+// wave 1 runs it backward over points len-1..m+1,
+//     T_i = b_i - c_i D_(i+1),   D_i = T_i^-1 a_i,      M_i = -T_i^-1 c_i,
+// and the middle point takes both: T_m = b_m - a_m C_(m-1) - c_m D_(m+1), with
+// Ma = -T_m^-1 a_m and Mc = -T_m^-1 c_m.  Each half's chain is half the line.  Planes per
+// point: T^-1; M (Ma at m); -C or -D (Mc at m); the blocks to the previous and next line.
 template <typename LT, int NEQ>
-__global__ void __launch_bounds__(FW)
+__global__ void __launch_bounds__(2 * FW)
 k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ bjm,
             const LT *__restrict__ bjp, const LT *__restrict__ bkm, const LT *__restrict__ bkp,
             LT *__restrict__ opx, LT *__restrict__ ope)
 {
     constexpr int NB = NEQ * NEQ;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool eta = (int)blockIdx.x >= nx;
     // the xi lines are the longer chains: their waves win issue slots on a shared SIMD
     if (!eta) __builtin_amdgcn_s_setprio(2);
     const int l = eta ? (int)blockIdx.x - nx : (int)blockIdx.x;
     const int len = eta ? d.km : d.jm;
+    const int mid = len / 2;
     const long N = d.N;
     const long stride = eta ? d.jm : 1, base = eta ? (long)l : (long)l * d.jm;
     const LT *lm = eta ? bkm : bjm, *lp = eta ? bkp : bjp, *om = eta ? bjm : bkm,
              *opn = eta ? bjp : bkp;
     LT *out = eta ? ope : opx;
-    __shared__ LT sh[FW][3 * NB + 1];   // a, b, c of the chunk's points
-    __shared__ LT res[FW][NB + 1];      // C of the chunk's points
-    LT ra[NB], rb[NB], rc[NB], rom[NB], rop[NB];
+    __shared__ LT sh[2][FW][3 * NB + 1];   // per wave: carried-side, b, solve-side blocks
+    __shared__ LT res[2][FW][NB + 1];      // per wave: C (or D) of the chunk's points
+    __shared__ LT edge[2][NB];             // C_(m-1) and D_(m+1)
+    // wave 0 walks points 0, 1, .., mid-1; wave 1 walks len-1, len-2, .., mid+1
+    const bool fwd = wv == 0;
+    const int first = fwd ? 0 : len - 1, count = fwd ? mid : len - 1 - mid,
+              step = fwd ? 1 : -1;
+    // forward: the carried side is a (times C of the point before), the solved side c;
+    // backward the other way round
+    const LT *carried = fwd ? lm : lp, *solved = fwd ? lp : lm;
+    LT rcar[NB], rb[NB], rsol[NB], rom[NB], rop[NB];
     auto load = [&](int c0) {
-        const int i = c0 + lane;
-        if (i < len) {
+        const int u = c0 + lane;
+        if (u < count) {
+            const int i = first + step * u;
             const long p = base + stride * i;
+            const bool has_a = i > 0, has_c = i < len - 1;
+            const bool has_car = fwd ? has_a : has_c, has_sol = fwd ? has_c : has_a;
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
                 rb[e] = btc[e * N + p];
-                ra[e] = i > 0 ? lm[e * N + p] : (LT)0;
-                rc[e] = i < len - 1 ? lp[e * N + p] : (LT)0;
+                rcar[e] = has_car ? carried[e * N + p] : (LT)0;
+                rsol[e] = has_sol ? solved[e * N + p] : (LT)0;
                 rom[e] = om[e * N + p];
                 rop[e] = opn[e * N + p];
             }
         }
     };
-    // T = b - a C;  det(T) and adj(T) (T^-1 = adj / det)
-    auto pivot = [&](const LT *a, LT *t, const LT *C, LT *ad) -> LT {
+    // T = b - x C;  det(T) and adj(T) (T^-1 = adj / det)
+    auto pivot = [&](const LT *x, LT *t, const LT *C, LT *ad) -> LT {
 #pragma unroll
         for (int cc = 0; cc < NEQ; ++cc)
 #pragma unroll
             for (int r = 0; r < NEQ; ++r)
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
-                    t[r + NEQ * cc] = t[r + NEQ * cc] - a[r + NEQ * k] * C[k + NEQ * cc];
+                    t[r + NEQ * cc] = t[r + NEQ * cc] - x[r + NEQ * k] * C[k + NEQ * cc];
         if constexpr (NEQ == 3) return adj3(t, ad);
         ad[0] = (LT)1;
         return t[0];
@@ -214,67 +234,104 @@ k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ b
 #pragma unroll
     for (int e = 0; e < NB; ++e) C[e] = (LT)0;
     load(0);
-    for (int c0 = 0; c0 < len; c0 += FW) {
-        const int i = c0 + lane;
-        const long o = (long)l * len + i;
-        LT oa[NB], ob[NB], cin[NB];
+    // both waves make the same number of chunk steps (their barriers pair up); the shorter
+    // half idles through its last one
+    const int nchunk = len - 1 - mid > mid ? len - 1 - mid : mid;
+    for (int c0 = 0; c0 < nchunk; c0 += FW) {
+        const int u0 = c0 + lane;
+        const long o = (long)l * len + (first + step * u0);
+        LT ocar[NB], ob[NB], cin[NB];
 #pragma unroll
         for (int e = 0; e < NB; ++e) {
-            oa[e] = ra[e];
+            ocar[e] = rcar[e];
             ob[e] = rb[e];
             cin[e] = C[e];
         }
-        if (i < len) {
+        if (u0 < count) {
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
-                sh[lane][e] = ra[e];
-                sh[lane][NB + e] = rb[e];
-                sh[lane][2 * NB + e] = rc[e];
+                sh[wv][lane][e] = rcar[e];
+                sh[wv][lane][NB + e] = rb[e];
+                sh[wv][lane][2 * NB + e] = rsol[e];
                 out[(3 * NB + e) * N + o] = rom[e];
                 out[(4 * NB + e) * N + o] = rop[e];
             }
         }
         __syncthreads();
-        if (c0 + FW < len) load(c0 + FW);
-        const int nv = len - c0 < FW ? len - c0 : FW;
-        // the recurrence, in every lane: only C goes on to the next point
+        if (c0 + FW < count) load(c0 + FW);
+        const int nv = count - c0 < FW ? (count - c0 > 0 ? count - c0 : 0) : FW;
+        // the recurrence, in every lane of the wave: only C goes on to the next point
 #pragma unroll 2
         for (int u = 0; u < nv; ++u) {
-            LT a[NB], t[NB], c[NB], ad[NB], ac[NB];
+            LT x[NB], t[NB], c[NB], ad[NB], ac[NB];
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
-                a[e] = sh[u][e];
-                t[e] = sh[u][NB + e];
-                c[e] = sh[u][2 * NB + e];
+                x[e] = sh[wv][u][e];
+                t[e] = sh[wv][u][NB + e];
+                c[e] = sh[wv][u][2 * NB + e];
             }
-            const LT det = pivot(a, t, C, ad);
+            const LT det = pivot(x, t, C, ad);
             mm<LT, NEQ>(ad, c, ac);
             const LT rd = rcp_nr(det);
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
                 C[e] = ac[e] * rd;
-                res[u][e] = C[e];
+                res[wv][u][e] = C[e];
             }
         }
         __syncthreads();
         // each lane redoes its own point's pivot from the C entering it (the same operations
-        // as the recurrence, so the same values) and forms T^-1 and M = -T^-1 a
-        if (i < len) {
+        // as the recurrence, so the same values) and forms T^-1 and M = -T^-1 x
+        if (u0 < count) {
             LT cp[NB], ad[NB], bi[NB], m[NB];
 #pragma unroll
-            for (int e = 0; e < NB; ++e) cp[e] = lane > 0 ? res[lane - 1][e] : cin[e];
-            const LT rd = rcp_nr(pivot(oa, ob, cp, ad));
+            for (int e = 0; e < NB; ++e) cp[e] = lane > 0 ? res[wv][lane - 1][e] : cin[e];
+            const LT rd = rcp_nr(pivot(ocar, ob, cp, ad));
 #pragma unroll
             for (int e = 0; e < NB; ++e) bi[e] = ad[e] * rd;
-            mm<LT, NEQ>(bi, oa, m);
+            mm<LT, NEQ>(bi, ocar, m);
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
                 out[e * N + o] = bi[e];
                 out[(NB + e) * N + o] = -m[e];
-                out[(2 * NB + e) * N + o] = -res[lane][e];
+                out[(2 * NB + e) * N + o] = -res[wv][lane][e];
             }
         }
         __syncthreads();
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) edge[wv][e] = C[e];   // C_(m-1) / D_(m+1) (0 if empty)
+    }
+    __syncthreads();
+    // the middle point
+    if (threadIdx.x == 0) {
+        const long p = base + stride * mid, o = (long)l * len + mid;
+        LT a[NB], b[NB], c[NB], t[NB], ad[NB], u[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            b[e] = btc[e * N + p];
+            a[e] = mid > 0 ? lm[e * N + p] : (LT)0;
+            c[e] = mid < len - 1 ? lp[e * N + p] : (LT)0;
+            out[(3 * NB + e) * N + o] = om[e * N + p];
+            out[(4 * NB + e) * N + o] = opn[e * N + p];
+            t[e] = b[e];
+        }
+        pivot(a, t, edge[0], ad);        // b - a C_(m-1)
+        const LT det = pivot(c, t, edge[1], ad);   // - c D_(m+1)
+        const LT rd = rcp_nr(det);
+        LT bi[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) bi[e] = ad[e] * rd;
+        mm<LT, NEQ>(bi, a, u);
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            out[e * N + o] = bi[e];
+            out[(NB + e) * N + o] = -u[e];
+        }
+        mm<LT, NEQ>(bi, c, u);
+#pragma unroll
+        for (int e = 0; e < NB; ++e) out[(2 * NB + e) * N + o] = -u[e];
     }
 }
 
@@ -284,6 +341,95 @@ k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ b
 // s, dq: NEQ planes, line-major; op: the factor kernel's planes.  assign: dq = x (the SA's
 // lreltur); otherwise dq += underr (x - dq) (ref/lrelax.f:377-381).  couple = 0 leaves out
 // the neighbouring lines' terms: a plain solve of each line (the transposed sweeps).
+//
+// The solve follows the twisted factorisation (mid = len/2):
+//   phase 1  y_i = M_i y_(i-1) + T_i^-1 r_i  forward over i < mid, and
+//            z_i = M_i z_(i+1) + T_i^-1 r_i  backward over i > mid, both scans at once
+//            (each point's map is the identity in the scan of the other half); then
+//            x_mid = T^-1 r + Ma y_(mid-1) + Mc z_(mid+1) in the thread owning mid;
+//   phase 2  x_i = -C_i x_(i+1) + y_i  backward over i < mid, and
+//            x_i = -D_i x_(i-1) + z_i  forward over i > mid, the point mid entering both
+//            scans as the constant map x_mid.
+
+// one scan of the thread's PP points along the line (FWD: ascending, else descending) with
+// the value 0 entering the line: get(u, a) gives point u's map, put(u, x) receives the value
+// after it.  tot: the waves' totals in LDS (the caller synchronises before it is written
+// again).
+template <typename AT, int NEQ, int NT, int PP, bool FWD, class Get, class Put>
+SMAC_DEV void line_scan(Get get, Put put, AT (*tot)[NEQ * NEQ + NEQ], int lane, int wv)
+{
+    constexpr int NB = NEQ * NEQ, NW = NT / 64;
+    Aff<AT, NEQ> agg;
+#pragma unroll
+    for (int k = 0; k < PP; ++k) {
+        const int u = FWD ? k : PP - 1 - k;
+        Aff<AT, NEQ> a;
+        get(u, a);
+        if (k == 0) agg = a;
+        else {
+            aff_after(a, agg);
+            agg = a;
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Aff<AT, NEQ> o;
+        if constexpr (FWD) {
+            aff_shfl_up(o, agg, off);
+            if (lane >= off) aff_after(agg, o);
+        } else {
+            aff_shfl_down(o, agg, off);
+            if (lane + off < 64) aff_after(agg, o);
+        }
+    }
+    if (lane == (FWD ? 63 : 0)) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) tot[wv][e] = agg.m[e];
+#pragma unroll
+        for (int n = 0; n < NEQ; ++n) tot[wv][NB + n] = agg.v[n];
+    }
+    __syncthreads();
+    // the value entering this wave, then this thread
+    AT Y[NEQ], cur[NEQ];
+#pragma unroll
+    for (int n = 0; n < NEQ; ++n) Y[n] = (AT)0;
+    for (int k = 0; k < (FWD ? wv : NW - 1 - wv); ++k) {
+        const int w = FWD ? k : NW - 1 - k;
+        AT nY[NEQ];
+        aff_apply<AT, NEQ>(tot[w], tot[w] + NB, Y, nY);
+#pragma unroll
+        for (int n = 0; n < NEQ; ++n) Y[n] = nY[n];
+    }
+    AT ye[NEQ];
+    aff_apply<AT, NEQ>(agg.m, agg.v, Y, ye);
+#pragma unroll
+    for (int n = 0; n < NEQ; ++n) {
+        const AT nb = FWD ? __shfl_up(ye[n], 1, 64) : __shfl_down(ye[n], 1, 64);
+        cur[n] = lane == (FWD ? 0 : 63) ? Y[n] : nb;
+    }
+#pragma unroll
+    for (int k = 0; k < PP; ++k) {
+        const int u = FWD ? k : PP - 1 - k;
+        Aff<AT, NEQ> a;
+        get(u, a);
+        AT nx[NEQ];
+        aff_apply<AT, NEQ>(a.m, a.v, cur, nx);
+#pragma unroll
+        for (int n = 0; n < NEQ; ++n) cur[n] = nx[n];
+        put(u, cur);
+    }
+}
+
+template <typename AT, int NEQ>
+SMAC_DEV void aff_identity(Aff<AT, NEQ> &a)
+{
+#pragma unroll
+    for (int r = 0; r < NEQ; ++r) {
+#pragma unroll
+        for (int c = 0; c < NEQ; ++c) a.m[r + NEQ * c] = r == c ? (AT)1 : (AT)0;
+        a.v[r] = (AT)0;
+    }
+}
 
 template <typename LT, int NEQ, int NT, int PP, bool SEQ>
 __global__ void __launch_bounds__(NT)
@@ -295,6 +441,7 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
     constexpr int NB = NEQ * NEQ, NW = NT / 64;
     __shared__ AT lf[NW][NB + NEQ], lb[NW][NB + NEQ];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int mid = len / 2;
     const LT *pB = op, *pM = op + NB * N, *pC = op + 2 * NB * N, *pOm = op + 3 * NB * N,
              *pOp = op + 4 * NB * N;
     double dqprev[PP][NEQ];
@@ -304,10 +451,11 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
         if (l < 0) l += nline;
         const bool hm = couple && l > 0, hp = couple && l < nline - 1;
         const bool regm = SEQ && lprev == l - 1, regp = SEQ && lprev == l + 1;
-        Aff<AT, NEQ> mp[PP], agg;
-        AT cm[PP][NB];
-        double dold[PP][NEQ];
-        // ---- maps of this thread's points: y_i = M_i y_(i-1) + B_i^-1 r_i
+        AT mmat[PP][NB];               // M (Ma at mid)
+        AT bv[PP][NEQ];                // T^-1 r
+        AT w[PP][NEQ];                 // y (i < mid) or z (i > mid); x_mid; then x
+        AT ymid[NEQ], zmid[NEQ];       // y_(mid-1), z_(mid+1) at mid
+        // ---- this thread's points: T^-1 r, M, -C / -D
 #pragma unroll
         for (int u = 0; u < PP; ++u) {
             const int i = t * PP + u;
@@ -315,10 +463,7 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                 const long o = (long)l * len + i;
                 double r[NEQ];
 #pragma unroll
-                for (int n = 0; n < NEQ; ++n) {
-                    r[n] = s[n * N + o];
-                    dold[u][n] = dq[n * N + o];
-                }
+                for (int n = 0; n < NEQ; ++n) r[n] = s[n * N + o];
                 if (hm) {
                     double x[NEQ];
 #pragma unroll
@@ -343,144 +488,141 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                         r[m] = r[m] - b;
                     }
                 }
-                {
-                    AT bi[NB], rr[NEQ];
+                AT bi[NB], rr[NEQ];
 #pragma unroll
-                    for (int e = 0; e < NB; ++e) bi[e] = pB[e * N + o];
+                for (int e = 0; e < NB; ++e) bi[e] = pB[e * N + o];
 #pragma unroll
-                    for (int n = 0; n < NEQ; ++n) rr[n] = (AT)r[n];
+                for (int n = 0; n < NEQ; ++n) rr[n] = (AT)r[n];
 #pragma unroll
-                    for (int m = 0; m < NEQ; ++m) {
-                        AT acc = bi[m] * rr[0];
+                for (int m = 0; m < NEQ; ++m) {
+                    AT acc = bi[m] * rr[0];
 #pragma unroll
-                        for (int n = 1; n < NEQ; ++n) acc = acc + bi[m + NEQ * n] * rr[n];
-                        mp[u].v[m] = acc;
-                    }
+                    for (int n = 1; n < NEQ; ++n) acc = acc + bi[m + NEQ * n] * rr[n];
+                    bv[u][m] = acc;
                 }
 #pragma unroll
-                for (int e = 0; e < NB; ++e) {
-                    mp[u].m[e] = pM[e * N + o];
-                    cm[u][e] = pC[e * N + o];
-                }
+                for (int e = 0; e < NB; ++e) mmat[u][e] = pM[e * N + o];
             } else {
 #pragma unroll
-                for (int e = 0; e < NB; ++e) {
-                    mp[u].m[e] = (AT)0;
-                    cm[u][e] = (AT)0;
+                for (int e = 0; e < NB; ++e) mmat[u][e] = (AT)0;
+#pragma unroll
+                for (int n = 0; n < NEQ; ++n) bv[u][n] = (AT)0;
+            }
+        }
+        // ---- phase 1: y forward over i < mid, z backward over i > mid
+        auto p1 = [&](bool fwd) {
+            return [&, fwd](int u, Aff<AT, NEQ> &a) {
+                const int i = t * PP + u;
+                if (i < len && (fwd ? i < mid : i > mid)) {
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) a.m[e] = mmat[u][e];
+#pragma unroll
+                    for (int n = 0; n < NEQ; ++n) a.v[n] = bv[u][n];
+                } else {
+                    aff_identity(a);
                 }
+            };
+        };
+        line_scan<AT, NEQ, NT, PP, true>(
+            p1(true),
+            [&](int u, const AT *x) {
+                const int i = t * PP + u;
 #pragma unroll
                 for (int n = 0; n < NEQ; ++n) {
-                    mp[u].v[n] = (AT)0;
-                    dold[u][n] = 0.0;
+                    if (i < mid) w[u][n] = x[n];
+                    if (i == mid) ymid[n] = x[n];
                 }
-            }
-            if (u == 0) agg = mp[0];
-            else {
-                Aff<AT, NEQ> a = mp[u];
-                aff_after(a, agg);
-                agg = a;
-            }
-        }
-        // ---- forward scan over the lanes, then over the waves
+            },
+            lf, lane, wv);
+        line_scan<AT, NEQ, NT, PP, false>(
+            p1(false),
+            [&](int u, const AT *x) {
+                const int i = t * PP + u;
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            Aff<AT, NEQ> o;
-            aff_shfl_up(o, agg, off);
-            if (lane >= off) aff_after(agg, o);
-        }
-        if (lane == 63) {
-#pragma unroll
-            for (int e = 0; e < NB; ++e) lf[wv][e] = agg.m[e];
-#pragma unroll
-            for (int n = 0; n < NEQ; ++n) lf[wv][NB + n] = agg.v[n];
-        }
-        __syncthreads();
-        AT Y[NEQ];
-#pragma unroll
-        for (int n = 0; n < NEQ; ++n) Y[n] = (AT)0;
-        for (int w = 0; w < wv; ++w) {
-            AT nY[NEQ];
-            aff_apply<AT, NEQ>(lf[w], lf[w] + NB, Y, nY);
-#pragma unroll
-            for (int n = 0; n < NEQ; ++n) Y[n] = nY[n];
-        }
-        AT ye[NEQ], cur[NEQ];
-        aff_apply<AT, NEQ>(agg.m, agg.v, Y, ye);
-#pragma unroll
-        for (int n = 0; n < NEQ; ++n) {
-            const AT up = __shfl_up(ye[n], 1, 64);
-            cur[n] = lane == 0 ? Y[n] : up;
-        }
-        AT y[PP][NEQ];
+                for (int n = 0; n < NEQ; ++n) {
+                    if (i > mid) w[u][n] = x[n];
+                    if (i == mid) zmid[n] = x[n];
+                }
+            },
+            lb, lane, wv);
+        // the -C / -D planes (Mc at mid), read only now: phase 1 does not need them
+        AT cm[PP][NB];
 #pragma unroll
         for (int u = 0; u < PP; ++u) {
-            AT nx[NEQ];
-            aff_apply<AT, NEQ>(mp[u].m, mp[u].v, cur, nx);
+            const int i = t * PP + u;
 #pragma unroll
-            for (int n = 0; n < NEQ; ++n) y[u][n] = cur[n] = nx[n];
+            for (int e = 0; e < NB; ++e)
+                cm[u][e] = i < len ? pC[e * N + (long)l * len + i] : (AT)0;
         }
-        // ---- backward: x_i = (-C_i) x_(i+1) + y_i, scanned from the end of the line
-        Aff<AT, NEQ> bagg;
+        // ---- the middle point: x_mid = T^-1 r + Ma y_(mid-1) + Mc z_(mid+1)
 #pragma unroll
-        for (int u = PP - 1; u >= 0; --u) {
-            Aff<AT, NEQ> a;
+        for (int u = 0; u < PP; ++u)
+            if (t * PP + u == mid) {
 #pragma unroll
-            for (int e = 0; e < NB; ++e) a.m[e] = cm[u][e];
+                for (int r = 0; r < NEQ; ++r) {
+                    AT acc = bv[u][r];
 #pragma unroll
-            for (int n = 0; n < NEQ; ++n) a.v[n] = y[u][n];
-            if (u == PP - 1) bagg = a;
-            else {
-                aff_after(a, bagg);
-                bagg = a;
+                    for (int k = 0; k < NEQ; ++k)
+                        acc = acc + mmat[u][r + NEQ * k] * ymid[k] + cm[u][r + NEQ * k] * zmid[k];
+                    w[u][r] = acc;
+                }
             }
-        }
+        __syncthreads();   // phase 1's wave totals have been read
+        // ---- phase 2: x backward over i < mid and forward over i > mid, from x_mid
+        auto p2 = [&](bool fwd) {
+            return [&, fwd](int u, Aff<AT, NEQ> &a) {
+                const int i = t * PP + u;
+                if (i < len && (fwd ? i > mid : i < mid)) {
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            Aff<AT, NEQ> o;
-            aff_shfl_down(o, bagg, off);
-            if (lane + off < 64) aff_after(bagg, o);
-        }
-        if (lane == 0) {
+                    for (int e = 0; e < NB; ++e) a.m[e] = cm[u][e];
 #pragma unroll
-            for (int e = 0; e < NB; ++e) lb[wv][e] = bagg.m[e];
+                    for (int n = 0; n < NEQ; ++n) a.v[n] = w[u][n];
+                } else if (i == mid) {
 #pragma unroll
-            for (int n = 0; n < NEQ; ++n) lb[wv][NB + n] = bagg.v[n];
-        }
-        __syncthreads();
-        AT X[NEQ];
+                    for (int e = 0; e < NB; ++e) a.m[e] = (AT)0;
 #pragma unroll
-        for (int n = 0; n < NEQ; ++n) X[n] = (AT)0;
-        for (int w = NW - 1; w > wv; --w) {
-            AT nX[NEQ];
-            aff_apply<AT, NEQ>(lb[w], lb[w] + NB, X, nX);
+                    for (int n = 0; n < NEQ; ++n) a.v[n] = w[u][n];
+                } else {
+                    aff_identity(a);
+                }
+            };
+        };
+        // (each scan's own point reads w before it writes it there; the other half's points
+        // are identities, so x overwrites y / z in place)
+        line_scan<AT, NEQ, NT, PP, false>(
+            p2(false),
+            [&](int u, const AT *x) {
+                const int i = t * PP + u;
 #pragma unroll
-            for (int n = 0; n < NEQ; ++n) X[n] = nX[n];
-        }
-        AT xs[NEQ];
-        aff_apply<AT, NEQ>(bagg.m, bagg.v, X, xs);
+                for (int n = 0; n < NEQ; ++n)
+                    if (i < mid) w[u][n] = x[n];
+            },
+            lb, lane, wv);
+        line_scan<AT, NEQ, NT, PP, true>(
+            p2(true),
+            [&](int u, const AT *x) {
+                const int i = t * PP + u;
 #pragma unroll
-        for (int n = 0; n < NEQ; ++n) {
-            const AT dn = __shfl_down(xs[n], 1, 64);
-            cur[n] = lane == 63 ? X[n] : dn;
-        }
+                for (int n = 0; n < NEQ; ++n)
+                    if (i > mid) w[u][n] = x[n];
+            },
+            lf, lane, wv);
 #pragma unroll
-        for (int u = PP - 1; u >= 0; --u) {
-            AT nx[NEQ];
-            aff_apply<AT, NEQ>(cm[u], y[u], cur, nx);
-#pragma unroll
-            for (int n = 0; n < NEQ; ++n) cur[n] = nx[n];
+        for (int u = 0; u < PP; ++u) {
             const int i = t * PP + u;
             if (i < len) {
                 const long o = (long)l * len + i;
 #pragma unroll
                 for (int n = 0; n < NEQ; ++n) {
-                    const double xd = (double)cur[n];
-                    const double nv = assign ? xd : dold[u][n] + underr * (xd - dold[u][n]);
+                    const double xd = (double)w[u][n];
+                    const double dold = assign ? 0.0 : dq[n * N + o];
+                    const double nv = assign ? xd : dold + underr * (xd - dold);
                     dq[n * N + o] = nv;
                     dqprev[u][n] = nv;
                 }
             }
         }
+        __syncthreads();   // phase 2's wave totals have been read before the next line
         lprev = l;
     }
 }
@@ -588,7 +730,7 @@ void launch_lr_factor3(hipStream_t sm, const Dims &d, bool xi, bool eta, const L
 {
     const int nx = xi ? d.km : 0, ne = eta ? d.jm : 0;
     if (nx + ne > 0)
-        hipLaunchKernelGGL((k_lr_factor<LT, 3>), dim3(nx + ne), dim3(FW), 0, sm, d, nx, btc, bjm,
+        hipLaunchKernelGGL((k_lr_factor<LT, 3>), dim3(nx + ne), dim3(2 * FW), 0, sm, d, nx, btc, bjm,
                            bjp, bkm, bkp, opx, ope);
 }
 
@@ -720,7 +862,7 @@ void launch_sa_lreltur_scan(hipStream_t sm, const Dims &d, int ntjsp, const Wake
     // the zebra order (that of the mean flow) a pass takes the even lines, the odd lines, and
     // then line k = 1 again.
     hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, sm, a.ds, d.N);
-    hipLaunchKernelGGL((k_lr_factor<double, 1>), dim3(d.km), dim3(FW), 0, sm, d, d.km, a.stc,
+    hipLaunchKernelGGL((k_lr_factor<double, 1>), dim3(d.km), dim3(2 * FW), 0, sm, d, d.km, a.stc,
                        a.sjm, a.sjp, a.skm, a.skp, ws.sa_op, (double *)nullptr);
     for (int pass = 1; pass <= ntjsp; ++pass) {
         if (ws.zebra) {
