@@ -918,10 +918,14 @@ void adj_build(AdjCtx &c)
     hipLaunchKernelGGL(k_adj_transpose_blocks, g3, dim3(256), 0, c.sm, d, c.blk[0], c.blk[1],
                        c.blk[2], c.blk[3], c.blk[4], c.tb[0], c.tb[1], c.tb[2], c.tb[3], c.tb[4]);
     if (c.lr) {
-        launch_lr_factor3<double>(c.sm, d, c.njsp > 0, c.nksp > 0, c.blk[0], c.blk[1], c.blk[2],
-                                  c.blk[3], c.blk[4], (double *)c.lr_f.opx, (double *)c.lr_f.ope);
-        launch_lr_factor3<double>(c.sm, d, c.njsp > 0, c.nksp > 0, c.tb[0], c.tb[1], c.tb[2],
-                                  c.tb[3], c.tb[4], (double *)c.lr_t.opx, (double *)c.lr_t.ope);
+        // the plain factorisation: factored once and swept thousands of times, the one-sided
+        // substitution (one scan each way) is the cheaper one here
+        launch_lr_factor3<double>(c.sm, d, c.njsp > 0, c.nksp > 0, false, c.blk[0], c.blk[1],
+                                  c.blk[2], c.blk[3], c.blk[4], (double *)c.lr_f.opx,
+                                  (double *)c.lr_f.ope);
+        launch_lr_factor3<double>(c.sm, d, c.njsp > 0, c.nksp > 0, false, c.tb[0], c.tb[1],
+                                  c.tb[2], c.tb[3], c.tb[4], (double *)c.lr_t.opx,
+                                  (double *)c.lr_t.ope);
     }
     for (int o = 0; o < 2; ++o)
         hipLaunchKernelGGL(k_adj_axpby, dim3(nblk(3 * N)), dim3(256), 0, c.sm, 3 * N, 0.0,
@@ -1002,7 +1006,7 @@ void adj_apply_prec(AdjCtx &c, const double *in, double *out)
     if (c.lr)
         launch_lrelax_scan<double>(c.sm, d, c.dp, c.njsp, c.nksp, 1, true, none, c.q, out, c.t1,
                                    c.blk[0], c.blk[1], c.blk[2], c.blk[3], c.blk[4], c.lr_f,
-                                   false);
+                                   false, 0);
     else
         launch_lusgs3<double>(c.sm, d, c.ws, none, c.q, out, c.t1, c.blk[0], c.blk[1], c.blk[2],
                               c.blk[3], c.blk[4], nullptr, false);

@@ -172,7 +172,7 @@ constexpr int FW = 64;   // points per chunk = lanes of the wave
 // point: T^-1; M (Ma at m); -C or -D (Mc at m); the blocks to the previous and next line.
 template <typename LT, int NEQ>
 __global__ void __launch_bounds__(2 * FW)
-k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ bjm,
+k_lr_factor(Dims d, int nx, int twist, const LT *__restrict__ btc, const LT *__restrict__ bjm,
             const LT *__restrict__ bjp, const LT *__restrict__ bkm, const LT *__restrict__ bkp,
             LT *__restrict__ opx, LT *__restrict__ ope)
 {
@@ -183,7 +183,9 @@ k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ b
     if (!eta) __builtin_amdgcn_s_setprio(2);
     const int l = eta ? (int)blockIdx.x - nx : (int)blockIdx.x;
     const int len = eta ? d.km : d.jm;
-    const int mid = len / 2;
+    // twist = 0: the plain (one-sided) factorisation, forward over the whole line (wave 0;
+    // launched with one wave)
+    const int mid = twist ? len / 2 : len;
     const long N = d.N;
     const long stride = eta ? d.jm : 1, base = eta ? (long)l : (long)l * d.jm;
     const LT *lm = eta ? bkm : bjm, *lp = eta ? bkp : bjp, *om = eta ? bjm : bkm,
@@ -194,7 +196,7 @@ k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ b
     __shared__ LT edge[2][NB];             // C_(m-1) and D_(m+1)
     // wave 0 walks points 0, 1, .., mid-1; wave 1 walks len-1, len-2, .., mid+1
     const bool fwd = wv == 0;
-    const int first = fwd ? 0 : len - 1, count = fwd ? mid : len - 1 - mid,
+    const int first = fwd ? 0 : len - 1, count = fwd ? mid : (len - 1 - mid > 0 ? len - 1 - mid : 0),
               step = fwd ? 1 : -1;
     // forward: the carried side is a (times C of the point before), the solved side c;
     // backward the other way round
@@ -236,7 +238,7 @@ k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ b
     load(0);
     // both waves make the same number of chunk steps (their barriers pair up); the shorter
     // half idles through its last one
-    const int nchunk = len - 1 - mid > mid ? len - 1 - mid : mid;
+    const int nchunk = twist && len - 1 - mid > mid ? len - 1 - mid : mid;
     for (int c0 = 0; c0 < nchunk; c0 += FW) {
         const int u0 = c0 + lane;
         const long o = (long)l * len + (first + step * u0);
@@ -299,6 +301,7 @@ k_lr_factor(Dims d, int nx, const LT *__restrict__ btc, const LT *__restrict__ b
         }
         __syncthreads();
     }
+    if (!twist) return;
     if (lane == 0) {
 #pragma unroll
         for (int e = 0; e < NB; ++e) edge[wv][e] = C[e];   // C_(m-1) / D_(m+1) (0 if empty)
@@ -431,7 +434,7 @@ SMAC_DEV void aff_identity(Aff<AT, NEQ> &a)
     }
 }
 
-template <typename LT, int NEQ, int NT, int PP, bool SEQ>
+template <typename LT, int NEQ, int NT, int PP, bool SEQ, bool TWIST>
 __global__ void __launch_bounds__(NT)
 k_lr_sweep(int nline, int len, long N, int count, int first, int step, double underr,
            int assign, int couple, double *__restrict__ dq, const double *__restrict__ s,
@@ -441,7 +444,9 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
     constexpr int NB = NEQ * NEQ, NW = NT / 64;
     __shared__ AT lf[NW][NB + NEQ], lb[NW][NB + NEQ];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int mid = len / 2;
+    // the plain factorisation: the forward half is the whole line, and the two scans of the
+    // other half (identities throughout) are left out
+    const int mid = TWIST ? len / 2 : len;
     const LT *pB = op, *pM = op + NB * N, *pC = op + 2 * NB * N, *pOm = op + 3 * NB * N,
              *pOp = op + 4 * NB * N;
     double dqprev[PP][NEQ];
@@ -534,17 +539,18 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                 }
             },
             lf, lane, wv);
-        line_scan<AT, NEQ, NT, PP, false>(
-            p1(false),
-            [&](int u, const AT *x) {
-                const int i = t * PP + u;
+        if constexpr (TWIST)
+            line_scan<AT, NEQ, NT, PP, false>(
+                p1(false),
+                [&](int u, const AT *x) {
+                    const int i = t * PP + u;
 #pragma unroll
-                for (int n = 0; n < NEQ; ++n) {
-                    if (i > mid) w[u][n] = x[n];
-                    if (i == mid) zmid[n] = x[n];
-                }
-            },
-            lb, lane, wv);
+                    for (int n = 0; n < NEQ; ++n) {
+                        if (i > mid) w[u][n] = x[n];
+                        if (i == mid) zmid[n] = x[n];
+                    }
+                },
+                lb, lane, wv);
         // the -C / -D planes (Mc at mid), read only now: phase 1 does not need them
         AT cm[PP][NB];
 #pragma unroll
@@ -554,10 +560,11 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
             for (int e = 0; e < NB; ++e)
                 cm[u][e] = i < len ? pC[e * N + (long)l * len + i] : (AT)0;
         }
-        // ---- the middle point: x_mid = T^-1 r + Ma y_(mid-1) + Mc z_(mid+1)
+        // ---- the middle point: x_mid = T^-1 r + Ma y_(mid-1) + Mc z_(mid+1) (the plain
+        // factorisation has none: there mid = len, past the line)
 #pragma unroll
         for (int u = 0; u < PP; ++u)
-            if (t * PP + u == mid) {
+            if (TWIST && t * PP + u == mid) {
 #pragma unroll
                 for (int r = 0; r < NEQ; ++r) {
                     AT acc = bv[u][r];
@@ -577,7 +584,7 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                     for (int e = 0; e < NB; ++e) a.m[e] = cm[u][e];
 #pragma unroll
                     for (int n = 0; n < NEQ; ++n) a.v[n] = w[u][n];
-                } else if (i == mid) {
+                } else if (TWIST && i == mid) {
 #pragma unroll
                     for (int e = 0; e < NB; ++e) a.m[e] = (AT)0;
 #pragma unroll
@@ -598,15 +605,16 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                     if (i < mid) w[u][n] = x[n];
             },
             lb, lane, wv);
-        line_scan<AT, NEQ, NT, PP, true>(
-            p2(true),
-            [&](int u, const AT *x) {
-                const int i = t * PP + u;
+        if constexpr (TWIST)
+            line_scan<AT, NEQ, NT, PP, true>(
+                p2(true),
+                [&](int u, const AT *x) {
+                    const int i = t * PP + u;
 #pragma unroll
-                for (int n = 0; n < NEQ; ++n)
-                    if (i > mid) w[u][n] = x[n];
-            },
-            lf, lane, wv);
+                    for (int n = 0; n < NEQ; ++n)
+                        if (i > mid) w[u][n] = x[n];
+                },
+                lf, lane, wv);
 #pragma unroll
         for (int u = 0; u < PP; ++u) {
             const int i = t * PP + u;
@@ -685,36 +693,47 @@ __global__ void k_zero(double *__restrict__ a, long n)
 
 // ------------------------------------------------------------------------------------------
 
+// twist: the operand planes hold the twisted factorisation (the reference-order walk always
+// takes the plain one)
 template <typename LT, int NEQ, bool SEQ>
 static void sweep_launch(hipStream_t sm, int nline, int len, long N, int count, int first,
                          int step, double underr, int assign, double *dq, const double *s,
-                         const LT *op, int couple = 1)
+                         const LT *op, int couple, bool twist)
 {
     const dim3 g(SEQ ? 1 : count);
-#define SW(NT, PPV)                                                                            \
-    hipLaunchKernelGGL((k_lr_sweep<LT, NEQ, NT, PPV, SEQ>), g, dim3(NT), 0, sm, nline, len, N, \
-                       count, first, step, underr, assign, couple, dq, s, op)
-    if (len <= 256) SW(256, 1);
-    else if (len <= 512) SW(512, 1);
-    else if (len <= 1024) SW(512, 2);
-    else SW(1024, 2);
+#define SW(NT, PPV, TW)                                                                         \
+    hipLaunchKernelGGL((k_lr_sweep<LT, NEQ, NT, PPV, SEQ, TW>), g, dim3(NT), 0, sm, nline, len, \
+                       N, count, first, step, underr, assign, couple, dq, s, op)
+#define SWT(TW)                                                                                 \
+    do {                                                                                       \
+        if (len <= 256) SW(256, 1, TW);                                                        \
+        else if (len <= 512) SW(512, 1, TW);                                                   \
+        else if (len <= 1024) SW(512, 2, TW);                                                  \
+        else SW(1024, 2, TW);                                                                  \
+    } while (0)
+    if constexpr (SEQ) SWT(false);
+    else if (twist) SWT(true);
+    else SWT(false);
+#undef SWT
 #undef SW
 }
 
 // one sweep of lines (reference order: odd sweeps ascending, even descending, as
 // ref/lrelax.f:249-275; zebra: the lines of even index, then those of odd index)
 template <typename LT, int NEQ>
-static void sweep(hipStream_t sm, int nline, int len, long N, int nsweep, bool zebra,
+static void sweep(hipStream_t sm, int nline, int len, long N, int nsweep, bool zebra, bool twist,
                   double underr, double *dq, const double *s, const LT *op)
 {
     if (zebra) {
-        sweep_launch<LT, NEQ, false>(sm, nline, len, N, (nline + 1) / 2, 0, 2, underr, 0, dq, s, op);
-        sweep_launch<LT, NEQ, false>(sm, nline, len, N, nline / 2, 1, 2, underr, 0, dq, s, op);
+        sweep_launch<LT, NEQ, false>(sm, nline, len, N, (nline + 1) / 2, 0, 2, underr, 0, dq, s,
+                                     op, 1, twist);
+        sweep_launch<LT, NEQ, false>(sm, nline, len, N, nline / 2, 1, 2, underr, 0, dq, s, op, 1,
+                                     twist);
         return;
     }
     const bool rev = (nsweep / 2) * 2 == nsweep;
     sweep_launch<LT, NEQ, true>(sm, nline, len, N, nline, rev ? nline - 1 : 0, rev ? -1 : 1,
-                                underr, 0, dq, s, op);
+                                underr, 0, dq, s, op, 1, false);
 }
 
 long lr_scan_bytes(const Dims &d)
@@ -724,37 +743,40 @@ long lr_scan_bytes(const Dims &d)
 }
 
 template <typename LT>
-void launch_lr_factor3(hipStream_t sm, const Dims &d, bool xi, bool eta, const LT *btc,
-                       const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *opx,
-                       LT *ope)
+void launch_lr_factor3(hipStream_t sm, const Dims &d, bool xi, bool eta, bool twist,
+                       const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp,
+                       LT *opx, LT *ope)
 {
     const int nx = xi ? d.km : 0, ne = eta ? d.jm : 0;
     if (nx + ne > 0)
-        hipLaunchKernelGGL((k_lr_factor<LT, 3>), dim3(nx + ne), dim3(2 * FW), 0, sm, d, nx, btc, bjm,
-                           bjp, bkm, bkp, opx, ope);
+        hipLaunchKernelGGL((k_lr_factor<LT, 3>), dim3(nx + ne), dim3(twist ? 2 * FW : FW), 0, sm, d,
+                           nx, twist ? 1 : 0, btc, bjm, bjp, bkm, bkp, opx, ope);
 }
 
 template <typename LT>
 void launch_lrelax_scan(hipStream_t sm, const Dims &d, const DevParams &prm, int njsp, int nksp,
                         int nbcimp, bool zebra, const WakePairs &w, const double *q, double *dq,
                         double *s, const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm,
-                        const LT *bkp, const LrScan &ws, bool factor)
+                        const LT *bkp, const LrScan &ws, bool factor, int twist)
 {
     const long N = d.N;
     LT *opx = (LT *)ws.opx, *ope = (LT *)ws.ope;
+    // the twisted factorisation for the zebra passes (unless the caller says otherwise);
+    // the plain one for the reference-order walk
+    const bool tw = zebra && twist != 0;
     hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, sm, dq, 3 * N);
     // both directions' factorisations depend on the blocks only: one launch
     if (factor)
-        launch_lr_factor3<LT>(sm, d, njsp > 0, nksp > 0, btc, bjm, bjp, bkm, bkp, opx, ope);
+        launch_lr_factor3<LT>(sm, d, njsp > 0, nksp > 0, tw, btc, bjm, bjp, bkm, bkp, opx, ope);
     for (int nj = 1; nj <= njsp; ++nj) {
-        sweep<LT, 3>(sm, d.km, d.jm, N, nj, zebra, prm.underr, dq, s, opx);
+        sweep<LT, 3>(sm, d.km, d.jm, N, nj, zebra, tw, prm.underr, dq, s, opx);
         if ((nj / nbcimp) * nbcimp == nj) launch_wake(sm, d, 3, w, q, dq, s);
     }
     if (nksp > 0) {
         transpose(sm, s, ws.s_t, d.km, d.jm, N, 3);
         transpose(sm, dq, ws.dq_t, d.km, d.jm, N, 3);
         for (int nk = 1; nk <= nksp; ++nk) {
-            sweep<LT, 3>(sm, d.jm, d.km, N, nk, zebra, prm.underr, ws.dq_t, ws.s_t, ope);
+            sweep<LT, 3>(sm, d.jm, d.km, N, nk, zebra, tw, prm.underr, ws.dq_t, ws.s_t, ope);
             if ((nk / nbcimp) * nbcimp == nk && w.n > 0)
                 hipLaunchKernelGGL(k_wake_eta, dim3((w.n + 63) / 64), dim3(64), 0, sm, N, d.jm,
                                    d.km, 3, w.n, w.base, w.pairs, q, ws.dq_t, ws.s_t);
@@ -829,7 +851,8 @@ static void tsweep(hipStream_t sm, int nline, int len, long N, int nsweep, doubl
         for (int par = 1; par >= 0; --par) {
             const int count = par ? nline / 2 : (nline + 1) / 2;
             if (count > 0)
-                sweep_launch<LT, 3, false>(sm, nline, len, N, count, par, 2, 1.0, 1, z, a, op, 0);
+                sweep_launch<LT, 3, false>(sm, nline, len, N, count, par, 2, 1.0, 1, z, a, op, 0,
+                                           false);
             hipLaunchKernelGGL(k_lr_tspread<LT>, dim3(g), dim3(256), 0, sm, nline, len, N, par, w,
                                a, sbar, (const double *)z, op);
         }
@@ -862,25 +885,26 @@ void launch_sa_lreltur_scan(hipStream_t sm, const Dims &d, int ntjsp, const Wake
     // the zebra order (that of the mean flow) a pass takes the even lines, the odd lines, and
     // then line k = 1 again.
     hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, sm, a.ds, d.N);
-    hipLaunchKernelGGL((k_lr_factor<double, 1>), dim3(d.km), dim3(2 * FW), 0, sm, d, d.km, a.stc,
+    hipLaunchKernelGGL((k_lr_factor<double, 1>), dim3(d.km), dim3(ws.zebra ? 2 * FW : FW), 0, sm, d,
+                       d.km, ws.zebra ? 1 : 0, a.stc,
                        a.sjm, a.sjp, a.skm, a.skp, ws.sa_op, (double *)nullptr);
     for (int pass = 1; pass <= ntjsp; ++pass) {
         if (ws.zebra) {
             sweep_launch<double, 1, false>(sm, d.km, d.jm, d.N, (d.km + 1) / 2, 0, 2, 1.0, 1,
-                                           a.ds, a.sf, ws.sa_op);
+                                           a.ds, a.sf, ws.sa_op, 1, true);
             sweep_launch<double, 1, false>(sm, d.km, d.jm, d.N, d.km / 2, 1, 2, 1.0, 1, a.ds,
-                                           a.sf, ws.sa_op);
+                                           a.sf, ws.sa_op, 1, true);
             sweep_launch<double, 1, false>(sm, d.km, d.jm, d.N, 1, 0, 1, 1.0, 1, a.ds, a.sf,
-                                           ws.sa_op);
+                                           ws.sa_op, 1, true);
         } else {
             sweep_launch<double, 1, true>(sm, d.km, d.jm, d.N, d.km + 1, 0, 1, 1.0, 1, a.ds, a.sf,
-                                          ws.sa_op);
+                                          ws.sa_op, 1, false);
         }
         launch_wake(sm, d, 1, w, a.anut, a.ds, a.sf);
     }
 }
 
-template void launch_lr_factor3<double>(hipStream_t, const Dims &, bool, bool, const double *,
+template void launch_lr_factor3<double>(hipStream_t, const Dims &, bool, bool, bool, const double *,
                                         const double *, const double *, const double *,
                                         const double *, double *, double *);
 template void launch_lrelax_scan_t<double>(hipStream_t, const Dims &, int, int, double,
@@ -890,10 +914,10 @@ template void launch_lrelax_scan<double>(hipStream_t, const Dims &, const DevPar
                                          int, bool, const WakePairs &, const double *, double *,
                                          double *, const double *, const double *,
                                          const double *, const double *, const double *,
-                                         const LrScan &, bool);
+                                         const LrScan &, bool, int);
 template void launch_lrelax_scan<float>(hipStream_t, const Dims &, const DevParams &, int, int,
                                         int, bool, const WakePairs &, const double *, double *,
                                         double *, const float *, const float *, const float *,
-                                        const float *, const float *, const LrScan &, bool);
+                                        const float *, const float *, const LrScan &, bool, int);
 
 }  // namespace smac

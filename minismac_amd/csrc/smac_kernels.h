@@ -61,17 +61,19 @@ struct LrScan {
     bool zebra;          // the SA's lines in the zebra order too
 };
 long lr_scan_bytes(const Dims &);
-// the line factorisations of both directions (either may be skipped) into opx / ope
+// the line factorisations of both directions (either may be skipped) into opx / ope,
+// twisted (from both ends) or plain
 template <typename LT>
-void launch_lr_factor3(hipStream_t, const Dims &, bool xi, bool eta, const LT *btc,
+void launch_lr_factor3(hipStream_t, const Dims &, bool xi, bool eta, bool twist, const LT *btc,
                        const LT *bjm, const LT *bjp, const LT *bkm, const LT *bkp, LT *opx,
                        LT *ope);
-// factor = false: the operand planes in the workspace are already those of these blocks
+// factor = false: the operand planes in the workspace are already those of these blocks.
+// twist: 0 the plain factorisation, else (default) the twisted one for the zebra order
 template <typename LT>
 void launch_lrelax_scan(hipStream_t, const Dims &, const DevParams &, int njsp, int nksp,
                         int nbcimp, bool zebra, const WakePairs &, const double *q, double *dq,
                         double *s, const LT *btc, const LT *bjm, const LT *bjp, const LT *bkm,
-                        const LT *bkp, const LrScan &, bool factor = true);
+                        const LT *bkp, const LrScan &, bool factor = true, int twist = 1);
 // out = L^T in, L the zebra sweeps (no wake injection) of the blocks factored in wt as their
 // transposed and shifted system; v: work vectors of 3 planes each
 struct LrTWork {
