@@ -35,6 +35,48 @@ def _same(a: Solver, b: Solver, tag: str):
         assert sa[k] == sb[k], f"{tag}: {k} {sa[k]!r} vs {sb[k]!r} ({a.case.name})"
 
 
+def test_multi_case_launches_match_eager_runs():
+    """Batches of LU-SGS cases run each kernel of a pass once for all cases after the first
+    pass (k_multi): mixed sizes, both precisions, cases with and without the SA model and an
+    inviscid one, band splits changed between calls (the kernel tables are rebuilt) and a
+    state change that sends a case back to its first pass."""
+    cases = [
+        naca_case(239, 61, m=0.02, p=0.3, t=0.10, alpha=6.0, reynum=2.0e5, beta=10.0, impsch=2),
+        naca_case(477, 121, m=0.04, p=0.4, t=0.12, alpha=4.0, reynum=1.0e6, beta=10.0,
+                  impsch=2, precision=SMAC_FP32_LHS),
+        naca_case(239, 61, m=0.05, p=0.5, t=0.15, alpha=-2.0, reynum=5.0e6, beta=10.0,
+                  impsch=2, iturb=0),
+        config_case(0, impsch=2, ivis=0, iturb=0),
+        config_case(0, impsch=2, precision=SMAC_FP32_LHS),
+    ]
+    eager = [Solver(c) for c in cases]
+    batched = [Solver(c) for c in cases]
+    try:
+        with Batch(batched) as b:
+            for call, npass in enumerate((2, 5, 3)):
+                print(f"multi-case batch call {call}: {npass} passes", flush=True)
+                if call == 1:
+                    for s in (eager[1], batched[1]):
+                        s.set_option("lusgs_rows", 16)
+                b.run(npass)
+                for s in eager:
+                    s.run(npass)
+                for se, sb in zip(eager, batched):
+                    _same(se, sb, f"multi call {call}")
+            # one case back to its first pass: that pass runs per case, then k_multi again
+            q0, a0, _ = eager[3].get_state()
+            for s in (eager[3], batched[3]):
+                s.set_state(q=q0, anut=a0, nt=0, resmax0=0.0, clift=0.0)
+            b.run(3)
+            for s in eager:
+                s.run(3)
+            for se, sb in zip(eager, batched):
+                _same(se, sb, "after reset")
+    finally:
+        for s in eager + batched:
+            s.close()
+
+
 def test_batch_runs_match_eager_runs():
     cases = _mixed_cases()
     eager = [Solver(c) for c in cases]
