@@ -162,8 +162,9 @@ def converged_run(cfg: int, maxit: int, drop: float, adjoint: bool) -> dict:
                     "adjoint_over_primal": round(ast["seconds"] / wall, 4),
                     "dcl_dalpha_per_deg": dcl[0], "dcd_dalpha_per_deg": dcd[0],
                     "dcl_dbumps": list(dcl[1:]), "dcd_dbumps": list(dcd[1:]),
-                    "note": "both adjoints (CL, CD) by GMRES(40) with the transposed LU-SGS "
-                            "update as right preconditioner, plus the gradients of 8 "
+                    "note": "both adjoints (CL, CD) by GMRES(100) with the transposed zebra "
+                            "line-relaxation update (the primal's own, exactly transposed) as "
+                            "right preconditioner, plus the gradients of 8 "
                             "Hicks-Henne bump modes (peaks 0.1, 0.3, 0.5, 0.7 on each side, "
                             "algebraic mesh deformation); primal = FP64 zebra passes from the "
                             "free stream to the same state"}

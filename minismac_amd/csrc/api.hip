@@ -52,6 +52,13 @@ struct smac_handle {
     double *lhs_copy[5] = {};
     SaArrays sa = {};
     DevStats *st = nullptr;
+    // Pinned host copy of the statistics and its sequence word, written by k_publish at the end
+    // of each pass whose statistics the host reads (host pointer; pin_dev is the device's).
+    struct Pinned {
+        DevStats s;
+        unsigned seq;
+    } *pin = nullptr, *pin_dev = nullptr;
+    unsigned pin_seq = 0;
     Scratch w = {};
     int *pairs = nullptr;
     WakePairs wake = {};
@@ -188,6 +195,7 @@ static void free_handle(smac_handle *h)
     if (h->stream_sa) (void)hipStreamDestroy(h->stream_sa);
     for (hipEvent_t e : {h->ev_q, h->ev_sa_read, h->ev_sa_done})
         if (e) (void)hipEventDestroy(e);
+    if (h->pin) (void)hipHostFree(h->pin);
     delete h;
 }
 
@@ -325,6 +333,13 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
                             &h->sa.vort, &h->sa.cu, &h->sa.ck, &h->sa.cv, &h->sa.fr, &h->sa.bprm};
     for (double **pp : sa_planes) TRY(dalloc(h, pp, N));
     TRY(dalloc(h, &h->st, 1));
+    if (hipHostMalloc((void **)&h->pin, sizeof *h->pin, hipHostMallocCoherent | hipHostMallocMapped) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void **)&h->pin_dev, h->pin, 0) != hipSuccess) {
+        free_handle(h);
+        return fail(SMAC_ERR_HIP, "hipHostMalloc / hipHostGetDevicePointer failed (statistics)");
+    }
+    std::memset((void *)h->pin, 0, sizeof *h->pin);
     if (p->impsch == 2) TRY(alloc_lusgs_ws(h, h->lusgs_rows));
     if (p->impsch == 1) {
         char *lb = nullptr;
@@ -602,14 +617,47 @@ static void accumulate_timing(smac_handle *h)
     h->t_n += 1;
 }
 
+// The statistics of the pass just enqueued, once it has run.  k_publish copies them into
+// pinned memory behind the pass and the host spins on its sequence word, so the device waits
+// for the host only as long as the host takes to see the word and enqueue the next pass (a
+// device-to-host copy and a stream synchronisation cost about 60 us a pass here).  The spin
+// asks the stream every 1024 turns, so a stream that failed or ended without the word ends it.
+// Timing mode reads them after a stream synchronisation (its events are read then too).
+static smac_status read_stats(smac_handle *h, DevStats &s)
+{
+    if (h->timing) {
+        HIP_TRY(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return SMAC_OK;
+    }
+    const unsigned want = ++h->pin_seq;
+    launch_publish(h->stream, h->st, &h->pin_dev->s, &h->pin_dev->seq, want);
+    HIP_TRY(hipGetLastError());
+    const unsigned *seq = &h->pin->seq;
+    for (unsigned turn = 1; __atomic_load_n(seq, __ATOMIC_ACQUIRE) != want; ++turn) {
+        if ((turn & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(h->stream);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) == want) break;
+                return fail(SMAC_ERR_HIP, "the pass statistics were not published");
+            }
+            if (e != hipErrorNotReady)
+                return fail(SMAC_ERR_HIP, std::string("pass statistics: ") + hipGetErrorString(e));
+        }
+        __builtin_ia32_pause();
+    }
+    std::memcpy(&s, (const void *)&h->pin->s, sizeof s);
+    return SMAC_OK;
+}
+
 smac_status smac_step(smac_handle *h, smac_iter_stats *st)
 {
     if (!h) return fail(SMAC_ERR_ARG, "handle is NULL");
     smac_status rc = enqueue_pass(h);
     if (rc) return rc;
     DevStats s;
-    HIP_TRY(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    rc = read_stats(h, s);
+    if (rc) return rc;
     accumulate_timing(h);
     if (h->nt == 1) h->rhs0 = s.rhsnorm;
     if (st) fill_stats(h, s, st);
@@ -633,8 +681,8 @@ smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *la
         smac_status rc = enqueue_pass(h);
         if (rc) return rc;
         if (drop > 0.0 || h->timing || n == niter - 1) {
-            HIP_TRY(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
-            HIP_TRY(hipStreamSynchronize(h->stream));
+            rc = read_stats(h, s);
+            if (rc) return rc;
             accumulate_timing(h);
             if (s.stall) {
                 if (done) *done = n + 1;

@@ -161,6 +161,29 @@ void launch_update_force(hipStream_t sm, const Dims &d, const DevParams &prm, co
                        w.flag, st);
 }
 
+// Copies a pass's statistics into pinned host memory, then writes `seq` there with a
+// system-scope release: a host thread that reads `seq` == val (acquire) then reads this pass's
+// statistics, with no copy engine or stream synchronisation in between.
+static constexpr int PUB_WORDS = (int)(sizeof(DevStats) / 4);
+static_assert(sizeof(DevStats) % 4 == 0 && PUB_WORDS <= 64, "one wavefront copies the statistics");
+
+__global__ void __launch_bounds__(64)
+k_publish(const DevStats *__restrict__ st, DevStats *host, unsigned *seq, unsigned val)
+{
+    const int t = (int)threadIdx.x;
+    if (t < PUB_WORDS)
+        __hip_atomic_store((unsigned *)host + t, ((const unsigned *)st)[t], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    __threadfence_system();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(seq, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_publish(hipStream_t sm, const DevStats *st, DevStats *host, unsigned *seq, unsigned val)
+{
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, sm, st, host, seq, val);
+}
+
 // ---- multi-case launches ------------------------------------------------------------------
 
 UpdArgs upd_args(const Dims &d, double *q, const double *dq, Scratch &w)

@@ -161,6 +161,8 @@ void launch_sa_update(hipStream_t, const Dims &, const DevParams &, Scratch &, D
 void launch_update_force(hipStream_t, const Dims &, const DevParams &, const Walls &,
                          const double *x, const double *y, const double *met, const double *dj,
                          double *q, const double *dq, Scratch &, DevStats *);
+// st -> *host (pinned, device-mapped), then *seq = val with a system-scope release
+void launch_publish(hipStream_t, const DevStats *st, DevStats *host, unsigned *seq, unsigned val);
 
 // adjoint.hip: discrete adjoint of the converged mean flow (CL, CD gradients)
 struct AdjCtx {
