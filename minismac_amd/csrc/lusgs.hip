@@ -550,7 +550,12 @@ SMAC_DEV T keep(T x)
 // RING_M of its steps in flight beyond the two the computing wave is about to read, so one
 // constant vmcnt wait per step suffices; one s_barrier per step paces the waves.  A slot holds
 // the step's operand groups (64 lanes x 16 bytes each) and the 32-byte hand-over slot.
+// The computing wave holds RING_PF steps' operands in registers: at the start of step h it
+// passes the barrier and issues the LDS reads of step h + 2, which then run under step h's
+// arithmetic (the reads are 17 instructions, more than the LDS counter can track, so issuing
+// them costs the wave a wait for the first ones; here that wait overlaps the arithmetic).
 constexpr int RING_LOADERS = 3;
+constexpr int RING_PF = 3;
 constexpr int RING_GROUP = BAND_LANES * 16;
 // FP32-LHS sweeps run in the scaled form (the blocks and the rhs premultiplied by the inverse
 // diagonal): they read no diagonal factors
@@ -558,7 +563,7 @@ template <typename T> constexpr bool scaled_form() { return sizeof(T) == 4; }
 // block groups a step reads: factors (FP64 only), j-neighbour and k-neighbour blocks
 template <typename T> constexpr int ring_blocks() { return scaled_form<T>() ? 2 : 3; }
 template <typename T> constexpr int ring_groups() { return ring_blocks<T>() * pairs_of<3, T>(9) + pairs_of<3, T>(3); }
-template <typename T> constexpr int ring_m() { return sizeof(T) == 8 ? 2 : 4; }
+template <typename T> constexpr int ring_m() { return sizeof(T) == 8 ? 2 : 3; }
 template <typename T> constexpr int ring_d() { return RING_LOADERS * ring_m<T>() + 2; }
 template <typename T> constexpr int ring_slots() { return ring_d<T>() + 1; }
 template <typename T> constexpr int ring_slot_bytes() { return ring_groups<T>() * RING_GROUP + 32; }
@@ -567,11 +572,17 @@ template <typename T> constexpr int ring_bytes() { return ring_slots<T>() * ring
 template <typename T> constexpr int ring_vm_per_step() { return ring_groups<T>() + 2; }
 static_assert((ring_m<double>() + 1) * ring_vm_per_step<double>() <= 63, "vmcnt budget");
 static_assert((ring_m<float>() + 1) * ring_vm_per_step<float>() <= 63, "vmcnt budget");
-static_assert(ring_d<float>() + 2 <= HP0 && ring_d<double>() + 2 <= HP0, "ring runs past padding");
+// the loaders run ring_d steps past the last chunk, which ends up to RING_PF - 1 steps past the band
+static_assert(ring_d<float>() + RING_PF <= HP0 && ring_d<double>() + RING_PF <= HP0,
+              "ring runs past padding");
 static_assert(ring_bytes<double>() <= 160 * 1024 && ring_bytes<float>() <= 160 * 1024, "LDS");
 
 // gfx9 s_waitcnt immediate: wait until at most n vector-memory operations are outstanding
 constexpr int vmcnt_imm(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+// ... and until no LDS read is outstanding.  Issued once before each of a sweep's step loops:
+// the wait analysis merges a loop's entry state into every iteration, and reads left in flight
+// by the code before the loop would otherwise cost a full LDS wait in every iteration.
+constexpr int LGKM_DRAIN = 15 | (7 << 4) | (0 << 8) | (3 << 14);
 
 template <int AUX>
 SMAC_DEV void ring_dma(rsrc_t r, unsigned char *lds, unsigned vo, unsigned so)
@@ -615,33 +626,54 @@ struct Band {
     unsigned vo, vp;       // byte offset of this lane's position: current / prefetch step
     unsigned ein_pf, eout; // soffset of the hand-over slots: prefetch step / current step
 
+    // RING: the first n elements of this lane's 16 bytes of group q.  A full group is one
+    // ds_read_b128; a partial one (a block's last group, the rhs group) reads only up to its
+    // last element, so no register receives a padding element: a register whose loaded value
+    // is dead gets reused at once, and writing it makes the wave wait for every LDS read in
+    // flight (a write-after-write hazard), twice a step.
+    template <int N>
+    SMAC_DEV void ring_elems(const unsigned char *b, int q, T *dst)
+    {
+        constexpr int PW = pair_width<3, T>();
+        const unsigned char *a = b + q * RING_GROUP;
+        if constexpr (N == PW) {
+            typedef T vec_t __attribute__((ext_vector_type(PW)));
+            const vec_t v = *(const vec_t *)a;
+#pragma unroll
+            for (int e = 0; e < PW; ++e) dst[e] = v[e];
+        } else {
+            static_assert(N >= 1 && N < PW, "partial group");
+            int e = 0;
+            if constexpr (N >= 2) {
+                typedef T vec2_t __attribute__((ext_vector_type(2)));
+                const vec2_t v = *(const vec2_t *)a;
+                dst[0] = v[0];
+                dst[1] = v[1];
+                e = 2;
+            }
+            if constexpr (N % 2 == 1) dst[e] = *(const T *)(a + e * sizeof(T));
+        }
+    }
+
     // RING: one step's operands from LDS (the hand-over slot is read as a broadcast)
     SMAC_DEV void ring_read(Slot<NEQ, T> &sl, unsigned off)
     {
         static_assert(NEQ == 3, "the ring serves the 3x3 sweeps");
         constexpr int GB = pairs_of<3, T>(9), GR = pairs_of<3, T>(3), PW = pair_width<3, T>();
-        // 16-byte vector reads from one base with immediate offsets (ds_read_b128)
-        typedef T vec_t __attribute__((ext_vector_type(PW)));
-        const vec_t *b = (const vec_t *)(ring + off + (threadIdx.x & 63) * 16);
-        constexpr int GV = RING_GROUP / 16;   // vectors per group
+        // reads from one base with immediate offsets
+        const unsigned char *b = ring + off + (threadIdx.x & 63) * 16;
         constexpr int NBK = ring_blocks<T>();
         T *dst[3] = {sl.f, sl.jn, sl.kn};
+        constexpr int LB = 9 - (GB - 1) * PW, LR = 3 - (GR - 1) * PW;   // last groups' elements
 #pragma unroll
-        for (int g = 0; g < NBK; ++g)
+        for (int g = 0; g < NBK; ++g) {
 #pragma unroll
-            for (int p = 0; p < GB; ++p) {
-                const vec_t v = b[(g * GB + p) * GV];
-#pragma unroll
-                for (int e = 0; e < PW; ++e)
-                    if (p * PW + e < 9) dst[g + 3 - NBK][p * PW + e] = v[e];
-            }
-#pragma unroll
-        for (int p = 0; p < GR; ++p) {
-            const vec_t v = b[(NBK * GB + p) * GV];
-#pragma unroll
-            for (int e = 0; e < PW; ++e)
-                if (p * PW + e < 3) sl.r[p * PW + e] = v[e];
+            for (int p = 0; p < GB - 1; ++p) ring_elems<PW>(b, g * GB + p, dst[g + 3 - NBK] + p * PW);
+            ring_elems<LB>(b, g * GB + GB - 1, dst[g + 3 - NBK] + (GB - 1) * PW);
         }
+#pragma unroll
+        for (int p = 0; p < GR - 1; ++p) ring_elems<PW>(b, NBK * GB + p, sl.r + p * PW);
+        ring_elems<LR>(b, NBK * GB + GR - 1, sl.r + (GR - 1) * PW);
         const unsigned long long *e = (const unsigned long long *)(ring + off + ring_groups<T>() * RING_GROUP);
         sl.e[0] = e[0];
         sl.e[1] = e[1];
@@ -669,9 +701,19 @@ struct Band {
     // JSEL: some lane may sit on the first (FWD) / last column, where the reference does not
     // form the j-neighbour term; KSEL: the band holds the first (FWD) / last grid row, where it
     // does not form the k-neighbour term.  Elsewhere both terms are formed unconditionally.
+    // One step on the operands in c.  RING: fill receives the operands of the step two ahead.
     template <bool JSEL, bool KSEL>
-    SMAC_DEV void step(Slot<NEQ, T> &c)
+    SMAC_DEV void step(Slot<NEQ, T> &c, Slot<NEQ, T> &fill)
     {
+        if constexpr (RING) {
+            // Every LDS read issued so far has landed (the slot a loader refills after this
+            // barrier was read two steps ago); past the barrier the loaders have the step two
+            // ahead in LDS.
+            __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
+            ring_barrier();
+            ring_read(fill, rnext);
+            rnext = rnext + ring_slot_bytes<T>() == ring_bytes<T>() ? 0u : rnext + ring_slot_bytes<T>();
+        }
         T nb[NEQ];
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) nb[m] = wave_shift<FWD>(own[m]);
@@ -802,14 +844,7 @@ struct Band {
         }
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) own[m] = o[m];
-        if constexpr (RING) {
-            // the loaders have the step two ahead in LDS once every wave is past this point
-            ring_barrier();
-            ring_read(c, rnext);
-            rnext = rnext + ring_slot_bytes<T>() == ring_bytes<T>() ? 0u : rnext + ring_slot_bytes<T>();
-        } else {
-            load(c, vp, ein_pf);
-        }
+        if constexpr (!RING) load(c, vp, ein_pf);
         constexpr int ED = FWD ? 32 : -32;
         h += FWD ? 1 : -1;
         vo += dstep;
@@ -822,8 +857,15 @@ struct Band {
     SMAC_DEV void chunks(int n)
     {
         for (int i = 0; i < n; ++i) {
+            if constexpr (RING) {
+                static_assert(PF == 3, "three register slots: in use, next, being read");
+                step<JSEL, KSEL>(s[0], s[2]);
+                step<JSEL, KSEL>(s[1], s[0]);
+                step<JSEL, KSEL>(s[2], s[1]);
+            } else {
 #pragma unroll
-            for (int u = 0; u < PF; ++u) step<JSEL, KSEL>(s[u]);
+                for (int u = 0; u < PF; ++u) step<JSEL, KSEL>(s[u], s[u]);
+            }
         }
     }
 };
@@ -892,7 +934,7 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
     // the first klast-k0+1 steps are the only ones where a lane meets the j boundary
     const int nch = (nsteps + PF - 1) / PF;
     if constexpr (RING) {
-        static_assert(PF == 2 && NEQ == 3, "the ring feeds a two-slot register ring");
+        static_assert(PF == RING_PF && NEQ == 3, "the ring feeds the register slots");
         __shared__ __attribute__((aligned(16))) unsigned char ring_lds[ring_bytes<T>()];
         const int wave = threadIdx.x >> 6;
         if (wave > 0) {
@@ -957,11 +999,14 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
     int nramp = (klast - k0 + 1 + PF - 1) / PF;
     nramp = nramp < nch ? nramp : nch;
     const bool krow = FWD ? k0 == 0 : klast == d.km - 1;
+    if constexpr (RING) __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
     if (krow) {
         B.template chunks<true, true>(nramp);
+        if constexpr (RING) __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
         B.template chunks<false, true>(nch - nramp);
     } else {
         B.template chunks<true, false>(nramp);
+        if constexpr (RING) __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
         B.template chunks<false, false>(nch - nramp);
     }
 }
@@ -991,7 +1036,7 @@ template <bool FWD, typename T>
 __global__ void __launch_bounds__(BAND_LANES * (1 + RING_LOADERS))
 k_band_ring(Dims d, LusgsWs w)
 {
-    band_run<3, 2, FWD, T, true>(d, w, (int)blockIdx.x);
+    band_run<3, RING_PF, FWD, T, true>(d, w, (int)blockIdx.x);
 }
 
 template <bool FWD, typename T>
@@ -1001,7 +1046,7 @@ k_band_ring_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map
     const int2 m = map[blockIdx.x];
     const Dims d = jobs[m.x].d;
     const LusgsWs w = jobs[m.x].w;
-    band_run<3, 2, FWD, T, true>(d, w, m.y);
+    band_run<3, RING_PF, FWD, T, true>(d, w, m.y);
 }
 
 // ------------------------------------------------------------------------------------------
