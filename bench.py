@@ -252,8 +252,10 @@ def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
                 BAND_FWD_BYTES["fp32" if bt["precision"] else "fp64"] * bt["points"]
                 / (bt["band_fwd_ms"] * 1e-3) / 1e9, 3) if bt["band_fwd_ms"] > 0 else None,
             "autotune_seconds": round(t_tune, 2),
-            "note": "one stream; the LU-SGS band sweeps of all cases in one launch per "
-                    "direction; sequential = the same cases run one after another with smac_run"}
+            "note": "one launch per kernel for all cases (the LU-SGS band sweeps of all cases "
+                    "in one launch per direction), the SA's launches on a second stream beside "
+                    "the mean flow's; sequential = the same cases run one after another with "
+                    "smac_run"}
 
 
 def main() -> int:

@@ -966,6 +966,8 @@ struct smac_batch {
     int band_group = -1;             // the group timed (the one with the most points)
     double band_ms = 0.0, band_pts = 0.0;
     hipEvent_t ev_join[BATCH_LANES] = {};
+    // multi-case passes: the SA launches run on lane[0] beside the mean flow's on `stream`
+    hipEvent_t ev_sa_read = nullptr, ev_sa_done = nullptr;
     BandGroup sa;                    // SA LU-SGS (neqs 1, FP64) of the LU-SGS cases
     BandGroup mean[2];               // mean-flow LU-SGS: FP64, FP32-LHS
     std::vector<LusgsWs> snap_mean, snap_sa;   // workspaces the tables were built from
@@ -1065,6 +1067,8 @@ static void free_batch(smac_batch *b)
         if (b->ev_join[k]) (void)hipEventDestroy(b->ev_join[k]);
     }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    for (hipEvent_t e : {b->ev_sa_read, b->ev_sa_done})
+        if (e) (void)hipEventDestroy(e);
     for (auto &e : b->ev_band)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1262,21 +1266,28 @@ static smac_status enqueue_batch_pass_multi(smac_batch *b, bool time_band)
 {
     hipStream_t sm = b->stream;
     for (smac_handle *h : b->cases) h->nt += 1;
-    // SA up to its band sweeps
-    multi_sa_deriv(sm, b->sa_deriv);
-    multi_sa_assemble(sm, b->sa_asm);
-    multi_sa_bc(sm, b->sa_bc);
-    multi_shear<double, 1>(sm, b->sa_shear);
+    // The SA runs on a second stream beside the mean flow, as in a single case's pass (the
+    // mean flow does not read the SA's results within a pass): it starts from the previous
+    // pass's q, and the mean flow's update, which overwrites q, waits until the SA has read it.
+    hipStream_t ss = b->lane[0];
+    HIP_TRY(hipEventRecord(b->ev_fork, sm));
+    HIP_TRY(hipStreamWaitEvent(ss, b->ev_fork, 0));
+    multi_sa_deriv(ss, b->sa_deriv);
+    multi_sa_assemble(ss, b->sa_asm);
+    multi_sa_bc(ss, b->sa_bc);
+    HIP_TRY(hipEventRecord(b->ev_sa_read, ss));   // the SA's last read of q
+    multi_shear<double, 1>(ss, b->sa_shear);
     BandGroup &gs = b->sa;
     if (gs.nblk) {
-        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true);
-        multi_mid<double, 1>(sm, b->sa_mid);
-        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false);
+        lusgs_band_multi<double, 1>(ss, gs.jobs, gs.map, gs.nblk, gs.pf, true);
+        multi_mid<double, 1>(ss, b->sa_mid);
+        lusgs_band_multi<double, 1>(ss, gs.jobs, gs.map, gs.nblk, gs.pf, false);
     }
-    multi_unshear<double, 1>(sm, b->sa_unshear);
-    multi_wake(sm, b->sa_wake);
-    multi_sa_update(sm, b->sa_upd);
-    multi_sa_turres(sm, b->sa_tur);
+    multi_unshear<double, 1>(ss, b->sa_unshear);
+    multi_wake(ss, b->sa_wake);
+    multi_sa_update(ss, b->sa_upd);
+    multi_sa_turres(ss, b->sa_tur);
+    HIP_TRY(hipEventRecord(b->ev_sa_done, ss));
     // mean flow up to its band sweeps
     multi_assemble<double>(sm, b->asm_[0]);
     multi_assemble<float>(sm, b->asm_[1]);
@@ -1306,8 +1317,10 @@ static smac_status enqueue_batch_pass_multi(smac_batch *b, bool time_band)
     multi_unshear<double, 3>(sm, b->unshear[0]);
     multi_unshear<float, 3>(sm, b->unshear[1]);
     multi_wake(sm, b->wake);
+    HIP_TRY(hipStreamWaitEvent(sm, b->ev_sa_read, 0));   // q is overwritten next
     multi_update(sm, b->upd);
     multi_force(sm, b->force);
+    HIP_TRY(hipStreamWaitEvent(sm, b->ev_sa_done, 0));   // the pass ends with both
     HIP_TRY(hipGetLastError());
     return SMAC_OK;
 }
@@ -1459,6 +1472,8 @@ smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out)
     }
     bool ok = hipStreamCreate(&b->stream) == hipSuccess &&
               hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_sa_read, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_sa_done, hipEventDisableTiming) == hipSuccess &&
               hipEventCreate(&b->ev_band[0]) == hipSuccess &&
               hipEventCreate(&b->ev_band[1]) == hipSuccess;
     for (int k = 0; k < BATCH_LANES && ok; ++k)
