@@ -43,6 +43,11 @@ struct smac_handle {
     // laminar viscosity, ref/visc.f:97), so it runs on a second stream beside it.
     hipStream_t stream_sa = nullptr;
     hipEvent_t ev_q = nullptr, ev_sa_read = nullptr, ev_sa_done = nullptr;
+    // In runs without a stop criterion a pass's wall forces run on stream_sa after the
+    // update, beside the next pass's assembly; the next pass's BC rows (which read clift)
+    // wait for ev_force.  force_side: the last pass left its forces there.
+    hipEvent_t ev_force = nullptr;
+    bool force_side = false;
     LusgsWs lws_sa = {};       // the SA's own LU-SGS workspace (neqs = 1)
     // grid (device)
     double *x = nullptr, *y = nullptr, *met = nullptr, *dj = nullptr;
@@ -193,7 +198,7 @@ static void free_handle(smac_handle *h)
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     if (h->stream_sa) (void)hipStreamDestroy(h->stream_sa);
-    for (hipEvent_t e : {h->ev_q, h->ev_sa_read, h->ev_sa_done})
+    for (hipEvent_t e : {h->ev_q, h->ev_sa_read, h->ev_sa_done, h->ev_force})
         if (e) (void)hipEventDestroy(e);
     if (h->pin) (void)hipHostFree(h->pin);
     delete h;
@@ -316,7 +321,8 @@ smac_status smac_create(const smac_params *p, int jmax, int kmax, const double *
     if (hipStreamCreate(&h->stream) != hipSuccess || hipStreamCreate(&h->stream_sa) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_q, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_sa_read, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&h->ev_sa_done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&h->ev_sa_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_force, hipEventDisableTiming) != hipSuccess) {
         free_handle(h);
         return fail(SMAC_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
@@ -525,7 +531,9 @@ static void enqueue_lrelax(smac_handle *h, hipStream_t sm)
 }
 
 // one pass of the time loop body: SA, assembly, BC rows, residual, solve, update + forces
-static smac_status enqueue_pass(smac_handle *h)
+// defer_force: leave this pass's wall forces on stream_sa, unjoined (smac_run between the
+// passes of a run without a stop criterion); the next pass joins them before its BC rows.
+static smac_status enqueue_pass(smac_handle *h, bool defer_force = false)
 {
     const Dims &d = h->d;
     const DevParams &dp = h->dp;
@@ -555,18 +563,23 @@ static smac_status enqueue_pass(smac_handle *h)
     }
     mark(h, 1);
     const bool f32 = h->p.precision == SMAC_FP32_LHS;
-    if (f32) {
-        // FP32-LHS: blocks stored (and factored, solved) in FP32 in the same allocations
-        float *fb[5] = {(float *)btc, (float *)bjm, (float *)bjp, (float *)bkm, (float *)bkp};
+    // FP32-LHS: blocks stored (and factored, solved) in FP32 in the same allocations
+    float *fb[5] = {(float *)btc, (float *)bjm, (float *)bjp, (float *)bkm, (float *)bkp};
+    if (f32)
         launch_assemble<float>(sm, d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2], fb[3],
                                fb[4]);
+    else
+        launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm, bkp);
+    if (h->force_side) {   // the far-field rows read the previous pass's clift
+        HIP_TRY(hipStreamWaitEvent(sm, h->ev_force, 0));
+        h->force_side = false;
+    }
+    if (f32)
         launch_bc<float>(sm, d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y,
                          h->met, h->q, h->s, fb[0], fb[1], fb[2], fb[3], fb[4]);
-    } else {
-        launch_assemble<double>(sm, d, dp, h->q, h->met, h->dj, h->s, btc, bjm, bjp, bkm, bkp);
+    else
         launch_bc<double>(sm, d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y,
                           h->met, h->q, h->s, btc, bjm, bjp, bkm, bkp);
-    }
     mark(h, 2);
     launch_resid(sm, d, h->nt, dp.beta, h->s, h->dj, h->w, h->st);
     if (h->keep_lhs)
@@ -585,8 +598,19 @@ static smac_status enqueue_pass(smac_handle *h)
     (void)scr1;
     mark(h, 4);
     if (side) HIP_TRY(hipStreamWaitEvent(sm, h->ev_sa_read, 0));   // q is overwritten next
-    launch_update_force(sm, d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq, h->w, h->st);
-    if (side) HIP_TRY(hipStreamWaitEvent(sm, h->ev_sa_done, 0));   // the pass ends with both
+    if (defer_force && !h->timing) {
+        // the forces read the updated q on stream_sa (after this pass's SA), unjoined
+        launch_update(sm, d, h->q, h->dq, h->w);
+        HIP_TRY(hipEventRecord(h->ev_q, sm));
+        HIP_TRY(hipStreamWaitEvent(h->stream_sa, h->ev_q, 0));
+        launch_force(h->stream_sa, d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->w, h->st);
+        HIP_TRY(hipEventRecord(h->ev_force, h->stream_sa));
+        h->force_side = true;
+    } else {
+        launch_update_force(sm, d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->dq, h->w,
+                            h->st);
+        if (side) HIP_TRY(hipStreamWaitEvent(sm, h->ev_sa_done, 0));   // the pass ends with both
+    }
     mark(h, 5);
     HIP_TRY(hipGetLastError());
     return SMAC_OK;
@@ -678,9 +702,17 @@ smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *la
     DevStats s = {};
     int n = 0;
     for (; n < niter; ++n) {
-        smac_status rc = enqueue_pass(h);
-        if (rc) return rc;
-        if (drop > 0.0 || h->timing || n == niter - 1) {
+        // a pass whose statistics nobody reads leaves its wall forces beside the next pass
+        const bool read = drop > 0.0 || h->timing || n == niter - 1;
+        smac_status rc = enqueue_pass(h, !read);
+        if (rc) {
+            if (h->force_side) {
+                (void)hipStreamWaitEvent(h->stream, h->ev_force, 0);
+                h->force_side = false;
+            }
+            return rc;
+        }
+        if (read) {
             rc = read_stats(h, s);
             if (rc) return rc;
             accumulate_timing(h);

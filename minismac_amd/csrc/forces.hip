@@ -150,15 +150,27 @@ k_force(Dims d, DevParams prm, Walls wl, const double *__restrict__ x, const dou
     b_force(d, prm, wl, x, y, met, dj, q, flag, st);
 }
 
-void launch_update_force(hipStream_t sm, const Dims &d, const DevParams &prm, const Walls &wl,
-                         const double *x, const double *y, const double *met, const double *dj,
-                         double *q, const double *dq, Scratch &w, DevStats *st)
+void launch_update(hipStream_t sm, const Dims &d, double *q, const double *dq, Scratch &w)
 {
     int nb = (int)((3 * d.N + 255) / 256);
     nb = nb > 4096 ? 4096 : nb;
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, sm, 3 * d.N, q, dq, w.flag);
+}
+
+void launch_force(hipStream_t sm, const Dims &d, const DevParams &prm, const Walls &wl,
+                  const double *x, const double *y, const double *met, const double *dj,
+                  const double *q, Scratch &w, DevStats *st)
+{
     hipLaunchKernelGGL(k_force, dim3(1), dim3(FORCE_THREADS), 0, sm, d, prm, wl, x, y, met, dj, q,
                        w.flag, st);
+}
+
+void launch_update_force(hipStream_t sm, const Dims &d, const DevParams &prm, const Walls &wl,
+                         const double *x, const double *y, const double *met, const double *dj,
+                         double *q, const double *dq, Scratch &w, DevStats *st)
+{
+    launch_update(sm, d, q, dq, w);
+    launch_force(sm, d, prm, wl, x, y, met, dj, q, w, st);
 }
 
 // Copies a pass's statistics into pinned host memory, then writes `seq` there with a

@@ -161,6 +161,12 @@ void launch_sa_update(hipStream_t, const Dims &, const DevParams &, Scratch &, D
 void launch_update_force(hipStream_t, const Dims &, const DevParams &, const Walls &,
                          const double *x, const double *y, const double *met, const double *dj,
                          double *q, const double *dq, Scratch &, DevStats *);
+// the two halves of launch_update_force: q += dq (raising the non-finite flag), then the
+// wall forces, CL/CD and clift (clearing the flag into st->nonfinite)
+void launch_update(hipStream_t, const Dims &, double *q, const double *dq, Scratch &);
+void launch_force(hipStream_t, const Dims &, const DevParams &, const Walls &, const double *x,
+                  const double *y, const double *met, const double *dj, const double *q,
+                  Scratch &, DevStats *);
 // st -> *host (pinned, device-mapped), then *seq = val with a system-scope release
 void launch_publish(hipStream_t, const DevStats *st, DevStats *host, unsigned *seq, unsigned val);
 
