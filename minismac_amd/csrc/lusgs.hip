@@ -119,6 +119,26 @@ SMAC_DEV void tile_to_sheared(const LusgsWs &w, int jm, int km, int j0, int k0,
     }
 }
 
+// one LDS plane of the tile into plane m of the sheared block at dst
+template <typename LT, int NEQ>
+SMAC_DEV void tile_plane_to_sheared(const LusgsWs &w, int jm, int km, int j0, int k0,
+                                    const double (*plane)[TPAD], LT *dst, int m)
+{
+    for (int e = threadIdx.x; e < (TJ + TK - 1) * TK; e += TILE_THREADS) {
+        const int kk = e % TK, hh = e / TK, jj = hh - kk;
+        const int j = j0 + jj, k = k0 + kk;
+        if (jj < 0 || jj >= TJ || j >= jm || k >= km) continue;
+        dst[EI<NEQ, LT>(w.ns, m, SX(w, j, k))] = (LT)plane[kk][jj];
+    }
+}
+
+// FP64 3x3 blocks fill 5 groups of 2, the last one half: its spare slot (plane 9) carries
+// what the LDS-ring sweeps need beside the block's ninth plane, so that a step reads 15 full
+// groups instead of 17 part-filled ones: f[8] beside bjm[8] and bjp[8], the rhs's third plane
+// beside bkm[8] (forward), the middle step's beside bkp[8] (backward).  The factors' fifth
+// group and the rhs's second stay written for the other readers.
+constexpr int SPARE = 9;
+
 template <typename LT, int NP>
 SMAC_DEV void tile_from_natural(const Dims &d, int j0, int k0, double (*lds)[TK][TPAD],
                                 const LT *src)
@@ -193,6 +213,11 @@ SMAC_DEV void b_shear(const Blk &bk, const Dims &d, const LusgsWs &w,
     }
     __syncthreads();
     if constexpr (!SCALED) tile_to_sheared<LT, NEQ, NB>(w, d.jm, d.km, j0, k0, lds, wl);
+    constexpr bool PACK = NEQ == 3 && sizeof(LT) == 8;
+    if constexpr (PACK) {
+        tile_plane_to_sheared<LT, NEQ>(w, d.jm, d.km, j0, k0, lds[8], wl + 1 * gstride, SPARE);
+        tile_plane_to_sheared<LT, NEQ>(w, d.jm, d.km, j0, k0, lds[8], wl + 3 * gstride, SPARE);
+    }
 
     // scaled form: this thread's points of the tile in LDS, NP planes (blocks: 3 columns)
     auto scale = [&](int np) {
@@ -231,6 +256,8 @@ SMAC_DEV void b_shear(const Blk &bk, const Dims &d, const LusgsWs &w,
         __syncthreads();
     }
     tile_to_sheared<LT, NEQ, NEQ>(w, d.jm, d.km, j0, k0, lds, (LT *)w.r);   // the solve's type
+    if constexpr (PACK)
+        tile_plane_to_sheared<LT, NEQ>(w, d.jm, d.km, j0, k0, lds[2], wl + 2 * gstride, SPARE);
 }
 
 template <typename LT, int NEQ>
@@ -308,7 +335,13 @@ SMAC_DEV void b_mid(const Blk &bk, const Dims &d, const LusgsWs &w)
             LT b33 = one / f[8] + f[2] * f[6] + f[5] * f[7];
             R[i0] = b11 * o0 + b12 * o1 + b13 * o2;
             R[i1] = b21 * o0 + b22 * o1 + b23 * o2;
-            R[i2] = b31 * o0 + b32 * o1 + b33 * o2;
+            const LT r2 = b31 * o0 + b32 * o1 + b33 * o2;
+            R[i2] = r2;
+            if constexpr (sizeof(LT) == 8) {
+                constexpr int PW = pair_width<NEQ, LT>();
+                const long gstride = (long)pairs_of<NEQ, LT>(NB) * w.ns * PW;
+                ((LT *)w.lhs)[4 * gstride + EI<NEQ, LT>(w.ns, SPARE, i)] = r2;   // beside bkp[8]
+            }
         } else {
             R[i] = R[i] / F[i];
         }
@@ -562,7 +595,12 @@ constexpr int RING_GROUP = BAND_LANES * 16;
 template <typename T> constexpr bool scaled_form() { return sizeof(T) == 4; }
 // block groups a step reads: factors (FP64 only), j-neighbour and k-neighbour blocks
 template <typename T> constexpr int ring_blocks() { return scaled_form<T>() ? 2 : 3; }
-template <typename T> constexpr int ring_groups() { return ring_blocks<T>() * pairs_of<3, T>(9) + pairs_of<3, T>(3); }
+// FP64: the packed step of 15 full groups (factors 0-3, j-block 0-4, k-block 0-4, rhs 0)
+template <typename T> constexpr bool ring_packed() { return sizeof(T) == 8; }
+template <typename T> constexpr int ring_groups()
+{
+    return ring_packed<T>() ? 4 + 5 + 5 + 1 : ring_blocks<T>() * pairs_of<3, T>(9) + pairs_of<3, T>(3);
+}
 template <typename T> constexpr int ring_m() { return sizeof(T) == 8 ? 2 : 3; }
 template <typename T> constexpr int ring_d() { return RING_LOADERS * ring_m<T>() + 2; }
 template <typename T> constexpr int ring_slots() { return ring_d<T>() + 1; }
@@ -662,18 +700,36 @@ struct Band {
         constexpr int GB = pairs_of<3, T>(9), GR = pairs_of<3, T>(3), PW = pair_width<3, T>();
         // reads from one base with immediate offsets
         const unsigned char *b = ring + off + (threadIdx.x & 63) * 16;
-        constexpr int NBK = ring_blocks<T>();
-        T *dst[3] = {sl.f, sl.jn, sl.kn};
-        constexpr int LB = 9 - (GB - 1) * PW, LR = 3 - (GR - 1) * PW;   // last groups' elements
+        if constexpr (ring_packed<T>()) {
+            // factors 0-7, j-block 0-7, (j8, f8), k-block 0-7, (k8, r2), (r0, r1)
 #pragma unroll
-        for (int g = 0; g < NBK; ++g) {
+            for (int p = 0; p < 4; ++p) ring_elems<2>(b, p, sl.f + 2 * p);
 #pragma unroll
-            for (int p = 0; p < GB - 1; ++p) ring_elems<PW>(b, g * GB + p, dst[g + 3 - NBK] + p * PW);
-            ring_elems<LB>(b, g * GB + GB - 1, dst[g + 3 - NBK] + (GB - 1) * PW);
+            for (int p = 0; p < 4; ++p) ring_elems<2>(b, 4 + p, sl.jn + 2 * p);
+            T t[2];
+            ring_elems<2>(b, 8, t);
+            sl.jn[8] = t[0];
+            sl.f[8] = t[1];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) ring_elems<2>(b, 9 + p, sl.kn + 2 * p);
+            ring_elems<2>(b, 13, t);
+            sl.kn[8] = t[0];
+            sl.r[2] = t[1];
+            ring_elems<2>(b, 14, sl.r);
+        } else {
+            constexpr int NBK = ring_blocks<T>();
+            T *dst[3] = {sl.f, sl.jn, sl.kn};
+            constexpr int LB = 9 - (GB - 1) * PW, LR = 3 - (GR - 1) * PW;   // last groups' elements
+#pragma unroll
+            for (int g = 0; g < NBK; ++g) {
+#pragma unroll
+                for (int p = 0; p < GB - 1; ++p) ring_elems<PW>(b, g * GB + p, dst[g + 3 - NBK] + p * PW);
+                ring_elems<LB>(b, g * GB + GB - 1, dst[g + 3 - NBK] + (GB - 1) * PW);
+            }
+#pragma unroll
+            for (int p = 0; p < GR - 1; ++p) ring_elems<PW>(b, NBK * GB + p, sl.r + p * PW);
+            ring_elems<LR>(b, NBK * GB + GR - 1, sl.r + (GR - 1) * PW);
         }
-#pragma unroll
-        for (int p = 0; p < GR - 1; ++p) ring_elems<PW>(b, NBK * GB + p, sl.r + p * PW);
-        ring_elems<LR>(b, NBK * GB + GR - 1, sl.r + (GR - 1) * PW);
         const unsigned long long *e = (const unsigned long long *)(ring + off + ring_groups<T>() * RING_GROUP);
         sl.e[0] = e[0];
         sl.e[1] = e[1];
@@ -948,15 +1004,30 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
                 if (st % RING_LOADERS != wl) return;
                 unsigned char *sb = ring_lds + (st % R) * SB;
                 const unsigned v = B.vo + (unsigned)(st * B.dstep);
+                if constexpr (ring_packed<T>()) {
+                    // factors' groups 0-3, both neighbour blocks' 0-4 (group 4 carries f8 / the
+                    // rhs's third plane in its spare slot), the rhs's group 0
 #pragma unroll
-                for (int g = 0; g < NBK; ++g)
+                    for (int p = 0; p < 4; ++p)
+                        ring_dma<0>(B.rF, sb + p * RING_GROUP, v, p * B.pstride);
 #pragma unroll
-                    for (int p = 0; p < GB; ++p)
-                        ring_dma<0>(rs[g + 3 - NBK], sb + (g * GB + p) * RING_GROUP, v,
-                                    p * B.pstride);
+                    for (int p = 0; p < 5; ++p)
+                        ring_dma<0>(B.rJ, sb + (4 + p) * RING_GROUP, v, p * B.pstride);
 #pragma unroll
-                for (int p = 0; p < GR; ++p)
-                    ring_dma<0>(B.rR, sb + (NBK * GB + p) * RING_GROUP, v, p * B.pstride);
+                    for (int p = 0; p < 5; ++p)
+                        ring_dma<0>(B.rK, sb + (9 + p) * RING_GROUP, v, p * B.pstride);
+                    ring_dma<0>(B.rR, sb + 14 * RING_GROUP, v, 0);
+                } else {
+#pragma unroll
+                    for (int g = 0; g < NBK; ++g)
+#pragma unroll
+                        for (int p = 0; p < GB; ++p)
+                            ring_dma<0>(rs[g + 3 - NBK], sb + (g * GB + p) * RING_GROUP, v,
+                                        p * B.pstride);
+#pragma unroll
+                    for (int p = 0; p < GR; ++p)
+                        ring_dma<0>(B.rR, sb + (NBK * GB + p) * RING_GROUP, v, p * B.pstride);
+                }
                 // the hand-over slot: lane 0 alone, so 32 bytes land at the slot's end
                 const unsigned es = ein0 + (unsigned)(st * dir * 32);
                 unsigned char *eb = sb + ring_groups<T>() * RING_GROUP;
