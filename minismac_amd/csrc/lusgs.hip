@@ -566,6 +566,31 @@ SMAC_DEV double wave_shift(double v)
                                             (unsigned long long)(unsigned)lo));
 }
 
+// As wave_shift, but the end lane (the one without a source) receives `in`: the incoming
+// hand-over value of a band's boundary row, which is that lane when the band has one.
+template <bool FWD>
+SMAC_DEV float wave_shift_in(float v, float in)
+{
+    constexpr int ctrl = FWD ? 0x138 : 0x130;
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, in),
+                                                                 __builtin_bit_cast(int, v), ctrl,
+                                                                 0xf, 0xf, false));
+}
+
+template <bool FWD>
+SMAC_DEV double wave_shift_in(double v, double in)
+{
+    constexpr int ctrl = FWD ? 0x138 : 0x130;
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long w = (unsigned long long)__double_as_longlong(in);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)w, (int)(unsigned)u, ctrl, 0xf, 0xf,
+                                               false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(w >> 32), (int)(unsigned)(u >> 32),
+                                               ctrl, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
 // Forces x to be computed where it stands: a term that is formed and then selected stays a
 // select instead of being sunk into a divergent branch around its computation.
 template <typename T>
@@ -757,8 +782,10 @@ struct Band {
     // JSEL: some lane may sit on the first (FWD) / last column, where the reference does not
     // form the j-neighbour term; KSEL: the band holds the first (FWD) / last grid row, where it
     // does not form the k-neighbour term.  Elsewhere both terms are formed unconditionally.
-    // One step on the operands in c.  RING: fill receives the operands of the step two ahead.
-    template <bool JSEL, bool KSEL>
+    // One step on the operands in c.  RING: fill receives the operands of the step two ahead;
+    // unless SEL the band's incoming row is the end lane of the shift, which then takes the
+    // hand-over value directly instead of through a select.
+    template <bool JSEL, bool KSEL, bool SEL = true>
     SMAC_DEV void step(Slot<NEQ, T> &c, Slot<NEQ, T> &fill)
     {
         if constexpr (RING) {
@@ -771,16 +798,22 @@ struct Band {
             rnext = rnext + ring_slot_bytes<T>() == ring_bytes<T>() ? 0u : rnext + ring_slot_bytes<T>();
         }
         T nb[NEQ];
-#pragma unroll
-        for (int m = 0; m < NEQ; ++m) nb[m] = wave_shift<FWD>(own[m]);
         bool empty = false;
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) empty = empty || c.e[m] == EDGE_EMPTY;
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(empty) != 0, 0))   // uniform; rare
             edge_wait<NEQ>(ein_ptr + (long)(h - kin + EPAD) * 4, c.e, broken, stall);
+        if constexpr (RING && !SEL) {
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m)
-            nb[m] = lane_in ? (T)__longlong_as_double((long long)c.e[m]) : nb[m];
+            for (int m = 0; m < NEQ; ++m)
+                nb[m] = wave_shift_in<FWD>(own[m], (T)__longlong_as_double((long long)c.e[m]));
+        } else {
+#pragma unroll
+            for (int m = 0; m < NEQ; ++m) nb[m] = wave_shift<FWD>(own[m]);
+#pragma unroll
+            for (int m = 0; m < NEQ; ++m)
+                nb[m] = lane_in ? (T)__longlong_as_double((long long)c.e[m]) : nb[m];
+        }
         const int j = h - k;
         // FWD: no j-term at j = 1 (bjm(1,k)*rhs(jmax,k-1) is an exact zero); backward: none at
         // j = jmax (bjp(jmax,k)*dq(1,k+1), likewise)
@@ -909,15 +942,15 @@ struct Band {
         eout += ED;
     }
 
-    template <bool JSEL, bool KSEL>
+    template <bool JSEL, bool KSEL, bool SEL = true>
     SMAC_DEV void chunks(int n)
     {
         for (int i = 0; i < n; ++i) {
             if constexpr (RING) {
                 static_assert(PF == 3, "three register slots: in use, next, being read");
-                step<JSEL, KSEL>(s[0], s[2]);
-                step<JSEL, KSEL>(s[1], s[0]);
-                step<JSEL, KSEL>(s[2], s[1]);
+                step<JSEL, KSEL, SEL>(s[0], s[2]);
+                step<JSEL, KSEL, SEL>(s[1], s[0]);
+                step<JSEL, KSEL, SEL>(s[2], s[1]);
             } else {
 #pragma unroll
                 for (int u = 0; u < PF; ++u) step<JSEL, KSEL>(s[u], s[u]);
@@ -1070,14 +1103,32 @@ SMAC_DEV void band_run(const Dims &d, const LusgsWs &w, int blk)
     int nramp = (klast - k0 + 1 + PF - 1) / PF;
     nramp = nramp < nch ? nramp : nch;
     const bool krow = FWD ? k0 == 0 : klast == d.km - 1;
-    if constexpr (RING) __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
-    if (krow) {
+    if constexpr (RING) {
+        // The band's incoming row is the shift's end lane (lane 0 forward; lane 63 backward,
+        // where only the head band, which has no incoming row, can be narrower than the wave)
+        // unless a backward band has fewer than 64 rows: then it selects the value (SEL).  A
+        // band without an incoming row takes the zeros of the dummy slots at its end lane, as
+        // the plain shift does.
+        const bool sel = has_in && !FWD && klast - k0 != BAND_LANES - 1;
+        __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
+        if (krow) {
+            B.template chunks<true, true, false>(nramp);
+            __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
+            B.template chunks<false, true, false>(nch - nramp);
+        } else if (!sel) {
+            B.template chunks<true, false, false>(nramp);
+            __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
+            B.template chunks<false, false, false>(nch - nramp);
+        } else {
+            B.template chunks<true, false, true>(nramp);
+            __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
+            B.template chunks<false, false, true>(nch - nramp);
+        }
+    } else if (krow) {
         B.template chunks<true, true>(nramp);
-        if constexpr (RING) __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
         B.template chunks<false, true>(nch - nramp);
     } else {
         B.template chunks<true, false>(nramp);
-        if constexpr (RING) __builtin_amdgcn_s_waitcnt(LGKM_DRAIN);
         B.template chunks<false, false>(nch - nramp);
     }
 }
