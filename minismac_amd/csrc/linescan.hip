@@ -443,10 +443,16 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
     using AT = LT;
     constexpr int NB = NEQ * NEQ, NW = NT / 64;
     __shared__ AT lf[NW][NB + NEQ], lb[NW][NB + NEQ];
+    __shared__ AT xch[2][NEQ];   // twisted: y_(mid-1) to the middle point, x_mid back
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    // the plain factorisation: the forward half is the whole line, and the two scans of the
-    // other half (identities throughout) are left out
+    // The plain factorisation: the forward half is the whole line.  The twisted one: the
+    // scans run over the line's points renumbered v = 0..len-1 as the first half in order
+    // (i = v < mid), the second half reversed (i = len - 1 - (v - mid)) and the middle point
+    // last, so that each phase is one scan: phase 1 forward, restarted where the second half
+    // begins (i = len - 1, whose map ignores what comes before it), and phase 2 backward from
+    // the middle point, the first half restarted from x_mid.
     const int mid = TWIST ? len / 2 : len;
+    auto phys = [&](int v) { return (!TWIST || v < mid) ? v : (v < len - 1 ? len - 1 - v + mid : mid); };
     const LT *pB = op, *pM = op + NB * N, *pC = op + 2 * NB * N, *pOm = op + 3 * NB * N,
              *pOp = op + 4 * NB * N;
     double dqprev[PP][NEQ];
@@ -458,14 +464,14 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
         const bool regm = SEQ && lprev == l - 1, regp = SEQ && lprev == l + 1;
         AT mmat[PP][NB];               // M (Ma at mid)
         AT bv[PP][NEQ];                // T^-1 r
-        AT w[PP][NEQ];                 // y (i < mid) or z (i > mid); x_mid; then x
-        AT ymid[NEQ], zmid[NEQ];       // y_(mid-1), z_(mid+1) at mid
+        AT w[PP][NEQ];                 // y (i < mid) or z (i > mid); z_(mid+1), then x_mid, at
+                                       // the middle point; then x
         // ---- this thread's points: T^-1 r, M, -C / -D
 #pragma unroll
         for (int u = 0; u < PP; ++u) {
-            const int i = t * PP + u;
-            if (i < len) {
-                const long o = (long)l * len + i;
+            const int v = t * PP + u;
+            if (v < len) {
+                const long o = (long)l * len + phys(v);
                 double r[NEQ];
 #pragma unroll
                 for (int n = 0; n < NEQ; ++n) r[n] = s[n * N + o];
@@ -514,112 +520,105 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                 for (int n = 0; n < NEQ; ++n) bv[u][n] = (AT)0;
             }
         }
-        // ---- phase 1: y forward over i < mid, z backward over i > mid
-        auto p1 = [&](bool fwd) {
-            return [&, fwd](int u, Aff<AT, NEQ> &a) {
-                const int i = t * PP + u;
-                if (i < len && (fwd ? i < mid : i > mid)) {
-#pragma unroll
-                    for (int e = 0; e < NB; ++e) a.m[e] = mmat[u][e];
-#pragma unroll
-                    for (int n = 0; n < NEQ; ++n) a.v[n] = bv[u][n];
-                } else {
-                    aff_identity(a);
-                }
-            };
-        };
+        // ---- phase 1: y forward over i < mid, z backward over i > mid, in one forward scan
         line_scan<AT, NEQ, NT, PP, true>(
-            p1(true),
+            [&](int u, Aff<AT, NEQ> &a) {
+                // every field written once, through selects (branches on the struct put it
+                // in scratch memory): past the line and at the middle point (which passes
+                // z_(mid+1) through) the identity; at the second half's first point a map
+                // that ignores what comes before it
+                const int v = t * PP + u;
+                const bool pass = v >= len || (TWIST && v == len - 1), rst = TWIST && v == mid;
+#pragma unroll
+                for (int r = 0; r < NEQ; ++r) {
+#pragma unroll
+                    for (int c = 0; c < NEQ; ++c)
+                        a.m[r + NEQ * c] = pass ? (r == c ? (AT)1 : (AT)0)
+                                                : rst ? (AT)0 : mmat[u][r + NEQ * c];
+                    a.v[r] = pass ? (AT)0 : bv[u][r];
+                }
+            },
             [&](int u, const AT *x) {
-                const int i = t * PP + u;
+                const int v = t * PP + u;
 #pragma unroll
                 for (int n = 0; n < NEQ; ++n) {
-                    if (i < mid) w[u][n] = x[n];
-                    if (i == mid) ymid[n] = x[n];
+                    if (v < len) w[u][n] = x[n];
+                    if (TWIST && v == mid - 1) xch[0][n] = x[n];
                 }
             },
             lf, lane, wv);
-        if constexpr (TWIST)
-            line_scan<AT, NEQ, NT, PP, false>(
-                p1(false),
-                [&](int u, const AT *x) {
-                    const int i = t * PP + u;
-#pragma unroll
-                    for (int n = 0; n < NEQ; ++n) {
-                        if (i > mid) w[u][n] = x[n];
-                        if (i == mid) zmid[n] = x[n];
-                    }
-                },
-                lb, lane, wv);
         // the -C / -D planes (Mc at mid), read only now: phase 1 does not need them
         AT cm[PP][NB];
 #pragma unroll
         for (int u = 0; u < PP; ++u) {
-            const int i = t * PP + u;
+            const int v = t * PP + u;
 #pragma unroll
             for (int e = 0; e < NB; ++e)
-                cm[u][e] = i < len ? pC[e * N + (long)l * len + i] : (AT)0;
+                cm[u][e] = v < len ? pC[e * N + (long)l * len + phys(v)] : (AT)0;
         }
+        __syncthreads();   // phase 1's wave totals have been read; y_(mid-1) is in xch
         // ---- the middle point: x_mid = T^-1 r + Ma y_(mid-1) + Mc z_(mid+1) (the plain
         // factorisation has none: there mid = len, past the line)
+        if constexpr (TWIST) {
 #pragma unroll
-        for (int u = 0; u < PP; ++u)
-            if (TWIST && t * PP + u == mid) {
+            for (int u = 0; u < PP; ++u)
+                if (t * PP + u == len - 1) {
+                    AT ymid[NEQ], zmid[NEQ];
+#pragma unroll
+                    for (int n = 0; n < NEQ; ++n) {
+                        ymid[n] = mid > 0 ? xch[0][n] : (AT)0;
+                        zmid[n] = len - 1 > mid ? w[u][n] : (AT)0;
+                    }
+#pragma unroll
+                    for (int r = 0; r < NEQ; ++r) {
+                        AT acc = bv[u][r];
+#pragma unroll
+                        for (int k = 0; k < NEQ; ++k)
+                            acc = acc + mmat[u][r + NEQ * k] * ymid[k] + cm[u][r + NEQ * k] * zmid[k];
+                        w[u][r] = acc;
+                        xch[1][r] = acc;
+                    }
+                }
+            __syncthreads();   // x_mid is in xch
+        }
+        // ---- phase 2: x backward over i < mid and forward over i > mid, from x_mid, in one
+        // backward scan: the middle point (last) gives x_mid, the second half follows it, and
+        // the first half's last point (v = mid - 1) restarts from x_mid
+        line_scan<AT, NEQ, NT, PP, false>(
+            [&](int u, Aff<AT, NEQ> &a) {
+                // as in phase 1, every field once: past the line the identity; at the middle
+                // point (0, x_mid); at the first half's last point (0, -C x_mid + y), a
+                // restart from x_mid; elsewhere (-C or -D, y or z)
+                const int v = t * PP + u;
+                const bool past = v >= len, midp = TWIST && v == len - 1;
+                const bool rst = TWIST && v == mid - 1;
+                AT xm[NEQ];
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k) xm[k] = TWIST ? xch[1][k] : (AT)0;
 #pragma unroll
                 for (int r = 0; r < NEQ; ++r) {
-                    AT acc = bv[u][r];
 #pragma unroll
-                    for (int k = 0; k < NEQ; ++k)
-                        acc = acc + mmat[u][r + NEQ * k] * ymid[k] + cm[u][r + NEQ * k] * zmid[k];
-                    w[u][r] = acc;
+                    for (int c = 0; c < NEQ; ++c)
+                        a.m[r + NEQ * c] = past ? (r == c ? (AT)1 : (AT)0)
+                                                : (midp || rst) ? (AT)0 : cm[u][r + NEQ * c];
+                    AT acc = w[u][r];
+#pragma unroll
+                    for (int k = 0; k < NEQ; ++k) acc = acc + cm[u][r + NEQ * k] * xm[k];
+                    a.v[r] = past ? (AT)0 : rst ? acc : w[u][r];
                 }
-            }
-        __syncthreads();   // phase 1's wave totals have been read
-        // ---- phase 2: x backward over i < mid and forward over i > mid, from x_mid
-        auto p2 = [&](bool fwd) {
-            return [&, fwd](int u, Aff<AT, NEQ> &a) {
-                const int i = t * PP + u;
-                if (i < len && (fwd ? i > mid : i < mid)) {
-#pragma unroll
-                    for (int e = 0; e < NB; ++e) a.m[e] = cm[u][e];
-#pragma unroll
-                    for (int n = 0; n < NEQ; ++n) a.v[n] = w[u][n];
-                } else if (TWIST && i == mid) {
-#pragma unroll
-                    for (int e = 0; e < NB; ++e) a.m[e] = (AT)0;
-#pragma unroll
-                    for (int n = 0; n < NEQ; ++n) a.v[n] = w[u][n];
-                } else {
-                    aff_identity(a);
-                }
-            };
-        };
-        // (each scan's own point reads w before it writes it there; the other half's points
-        // are identities, so x overwrites y / z in place)
-        line_scan<AT, NEQ, NT, PP, false>(
-            p2(false),
+            },
             [&](int u, const AT *x) {
-                const int i = t * PP + u;
+                const int v = t * PP + u;
 #pragma unroll
                 for (int n = 0; n < NEQ; ++n)
-                    if (i < mid) w[u][n] = x[n];
+                    if (v < len) w[u][n] = x[n];
             },
             lb, lane, wv);
-        if constexpr (TWIST)
-            line_scan<AT, NEQ, NT, PP, true>(
-                p2(true),
-                [&](int u, const AT *x) {
-                    const int i = t * PP + u;
-#pragma unroll
-                    for (int n = 0; n < NEQ; ++n)
-                        if (i > mid) w[u][n] = x[n];
-                },
-                lf, lane, wv);
 #pragma unroll
         for (int u = 0; u < PP; ++u) {
-            const int i = t * PP + u;
-            if (i < len) {
-                const long o = (long)l * len + i;
+            const int v = t * PP + u;
+            if (v < len) {
+                const long o = (long)l * len + phys(v);
 #pragma unroll
                 for (int n = 0; n < NEQ; ++n) {
                     const double xd = (double)w[u][n];
@@ -630,7 +629,7 @@ k_lr_sweep(int nline, int len, long N, int count, int first, int step, double un
                 }
             }
         }
-        __syncthreads();   // phase 2's wave totals have been read before the next line
+        __syncthreads();   // phase 2's wave totals and xch have been read before the next line
         lprev = l;
     }
 }
