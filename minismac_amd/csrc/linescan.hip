@@ -193,6 +193,7 @@ k_lr_factor(Dims d, int nx, int twist, const LT *__restrict__ btc, const LT *__r
     LT *out = eta ? ope : opx;
     __shared__ LT sh[2][FW][3 * NB + 1];   // per wave: carried-side, b, solve-side blocks
     __shared__ LT res[2][FW][NB + 1];      // per wave: C (or D) of the chunk's points
+    __shared__ LT cinl[2][NB];             // per wave: the C entering the chunk
     __shared__ LT edge[2][NB];             // C_(m-1) and D_(m+1)
     // wave 0 walks points 0, 1, .., mid-1; wave 1 walks len-1, len-2, .., mid+1
     const bool fwd = wv == 0;
@@ -202,6 +203,7 @@ k_lr_factor(Dims d, int nx, int twist, const LT *__restrict__ btc, const LT *__r
     // backward the other way round
     const LT *carried = fwd ? lm : lp, *solved = fwd ? lp : lm;
     LT rcar[NB], rb[NB], rsol[NB], rom[NB], rop[NB];
+    // a chunk's blocks of this direction (prefetched over the previous chunk's recurrence) ...
     auto load = [&](int c0) {
         const int u = c0 + lane;
         if (u < count) {
@@ -214,6 +216,17 @@ k_lr_factor(Dims d, int nx, int twist, const LT *__restrict__ btc, const LT *__r
                 rb[e] = btc[e * N + p];
                 rcar[e] = has_car ? carried[e * N + p] : (LT)0;
                 rsol[e] = has_sol ? solved[e * N + p] : (LT)0;
+            }
+        }
+    };
+    // ... and the other direction's two blocks, which pass through to the operand planes
+    // (loaded after the recurrence, so that they hold no registers during it)
+    auto load_o = [&](int c0) {
+        const int u = c0 + lane;
+        if (u < count) {
+            const long p = base + stride * (first + step * u);
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
                 rom[e] = om[e * N + p];
                 rop[e] = opn[e * N + p];
             }
@@ -236,18 +249,16 @@ k_lr_factor(Dims d, int nx, int twist, const LT *__restrict__ btc, const LT *__r
 #pragma unroll
     for (int e = 0; e < NB; ++e) C[e] = (LT)0;
     load(0);
+    load_o(0);
     // both waves make the same number of chunk steps (their barriers pair up); the shorter
     // half idles through its last one
     const int nchunk = twist && len - 1 - mid > mid ? len - 1 - mid : mid;
     for (int c0 = 0; c0 < nchunk; c0 += FW) {
         const int u0 = c0 + lane;
         const long o = (long)l * len + (first + step * u0);
-        LT ocar[NB], ob[NB], cin[NB];
+        if (lane == 0) {
 #pragma unroll
-        for (int e = 0; e < NB; ++e) {
-            ocar[e] = rcar[e];
-            ob[e] = rb[e];
-            cin[e] = C[e];
+            for (int e = 0; e < NB; ++e) cinl[wv][e] = C[e];
         }
         if (u0 < count) {
 #pragma unroll
@@ -262,32 +273,54 @@ k_lr_factor(Dims d, int nx, int twist, const LT *__restrict__ btc, const LT *__r
         __syncthreads();
         if (c0 + FW < count) load(c0 + FW);
         const int nv = count - c0 < FW ? (count - c0 > 0 ? count - c0 : 0) : FW;
-        // the recurrence, in every lane of the wave: only C goes on to the next point
-#pragma unroll 2
-        for (int u = 0; u < nv; ++u) {
-            LT x[NB], t[NB], c[NB], ad[NB], ac[NB];
+        // the recurrence, in every lane of the wave: only C goes on to the next point.  Two
+        // register sets alternate: the next point's blocks are read from LDS while this
+        // one's are used, so that their latency is off the chain; one lane keeps C for the
+        // per-lane pass below.
+        LT xa[NB], ta[NB], ca[NB], xb[NB], tb[NB], cb[NB];
+        auto rd3 = [&](int u, LT *x, LT *t, LT *c) {
+            const int q = u < FW ? u : FW - 1;   // past the chunk: read, never used
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
-                x[e] = sh[wv][u][e];
-                t[e] = sh[wv][u][NB + e];
-                c[e] = sh[wv][u][2 * NB + e];
+                x[e] = sh[wv][q][e];
+                t[e] = sh[wv][q][NB + e];
+                c[e] = sh[wv][q][2 * NB + e];
             }
+        };
+        auto rec = [&](const LT *x, LT *t, const LT *c, int u) {
+            LT ad[NB], ac[NB];
             const LT det = pivot(x, t, C, ad);
             mm<LT, NEQ>(ad, c, ac);
             const LT rd = rcp_nr(det);
 #pragma unroll
-            for (int e = 0; e < NB; ++e) {
-                C[e] = ac[e] * rd;
-                res[wv][u][e] = C[e];
+            for (int e = 0; e < NB; ++e) C[e] = ac[e] * rd;
+            if (lane == 0) {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) res[wv][u][e] = C[e];
+            }
+        };
+        rd3(0, xa, ta, ca);
+        for (int u = 0; u < nv; u += 2) {
+            rd3(u + 1, xb, tb, cb);
+            rec(xa, ta, ca, u);
+            if (u + 1 < nv) {
+                rd3(u + 2, xa, ta, ca);
+                rec(xb, tb, cb, u + 1);
             }
         }
         __syncthreads();
+        if (c0 + FW < count) load_o(c0 + FW);
         // each lane redoes its own point's pivot from the C entering it (the same operations
-        // as the recurrence, so the same values) and forms T^-1 and M = -T^-1 x
+        // as the recurrence, so the same values) and forms T^-1 and M = -T^-1 x; its blocks
+        // are still in the chunk's LDS
         if (u0 < count) {
-            LT cp[NB], ad[NB], bi[NB], m[NB];
+            LT cp[NB], ad[NB], bi[NB], m[NB], ocar[NB], ob[NB];
 #pragma unroll
-            for (int e = 0; e < NB; ++e) cp[e] = lane > 0 ? res[wv][lane - 1][e] : cin[e];
+            for (int e = 0; e < NB; ++e) {
+                cp[e] = lane > 0 ? res[wv][lane - 1][e] : cinl[wv][e];
+                ocar[e] = sh[wv][lane][e];
+                ob[e] = sh[wv][lane][NB + e];
+            }
             const LT rd = rcp_nr(pivot(ocar, ob, cp, ad));
 #pragma unroll
             for (int e = 0; e < NB; ++e) bi[e] = ad[e] * rd;
