@@ -326,7 +326,7 @@ k_assemble(Dims d, DevParams prm, const double *__restrict__ q, const double *__
            const double *__restrict__ dj, double *__restrict__ s, LT *__restrict__ btc,
            LT *__restrict__ bjm, LT *__restrict__ bjp, LT *__restrict__ bkm, LT *__restrict__ bkp)
 {
-    b_assemble<LT>(hw_blk(), d, prm, q, met, dj, s, btc, bjm, bjp, bkm, bkp);
+    b_assemble<LT>(xcd_blk(), d, prm, q, met, dj, s, btc, bjm, bjp, bkm, bkp);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -266,7 +266,7 @@ k_shear(Dims d, LusgsWs w, const double *__restrict__ rhs, const LT *__restrict_
         const LT *__restrict__ bjm, const LT *__restrict__ bkm, const LT *__restrict__ bjp,
         const LT *__restrict__ bkp, LT *__restrict__ fac)
 {
-    b_shear<LT, NEQ>(hw_blk(), d, w, rhs, btc, bjm, bkm, bjp, bkp, fac);
+    b_shear<LT, NEQ>(xcd_blk(), d, w, rhs, btc, bjm, bkm, bjp, bkp, fac);
 }
 
 // sheared -> natural (FP64) for the NEQ planes of a vector (dq, or the middle-step rhs)
@@ -299,7 +299,7 @@ template <int NEQ, typename T>
 __global__ void __launch_bounds__(TILE_THREADS)
 k_unshear(Dims d, LusgsWs w, const T *__restrict__ src, double *__restrict__ dst)
 {
-    b_unshear<NEQ, T>(hw_blk(), d, w, src, dst);
+    b_unshear<NEQ, T>(xcd_blk(), d, w, src, dst);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -111,7 +111,7 @@ __global__ void k_sa_deriv(Dims d, const double *__restrict__ q, const double *_
                            double *__restrict__ cu, double *__restrict__ ck,
                            double *__restrict__ cv, double *__restrict__ fr)
 {
-    b_sa_deriv(hw_blk(), d, q, met, dj, vort, cu, ck, cv, fr);
+    b_sa_deriv(xcd_blk(), d, q, met, dj, vort, cu, ck, cv, fr);
 }
 
 // tsafill loops 180-260 + tsaibt at one point (ref/tursa.f:597-917, 1353-1367)
@@ -285,7 +285,7 @@ __global__ void k_sa_assemble(Dims d, DevParams prm, const double *__restrict__ 
                               double *__restrict__ skm, double *__restrict__ skp,
                               double *__restrict__ stc, double *__restrict__ sf)
 {
-    b_sa_assemble(hw_blk(), d, prm, q, met, dj, anut, smin, use_smin, vort, cu, ck, cv, fr, sjm,
+    b_sa_assemble(xcd_blk(), d, prm, q, met, dj, anut, smin, use_smin, vort, cu, ck, cv, fr, sjm,
                   sjp, skm, skp, stc, sf);
 }
 

@@ -51,6 +51,20 @@ struct Blk {
 };
 SMAC_DEV Blk hw_blk() { return Blk{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}; }
 
+// The hardware's block index remapped so that each of the 8 XCDs (each with its own L2)
+// takes a contiguous run of the grid in row-major order: workgroup L of a launch runs on XCD
+// L % 8 as its (L / 8)-th, and gets logical block start(L % 8) + L / 8, a bijection onto the
+// grid.  Kernels whose points read neighbouring rows then find the rows shared with the
+// blocks beside them in their own XCD's L2.
+SMAC_DEV Blk xcd_blk()
+{
+    constexpr unsigned NX = 8;
+    const unsigned nb = gridDim.x * gridDim.y, L = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned q = nb / NX, r = nb % NX, x = L % NX;
+    const unsigned lg = x * q + (x < r ? x : r) + L / NX;
+    return Blk{lg % gridDim.x, lg / gridDim.x, gridDim.x, gridDim.y};
+}
+
 // One launch for many items (cases, or a case's BC regions) of one kernel: item i's own grid
 // is args[i].gx x args[i].gy blocks, and the launch's blocks are those grids one after the
 // other (start[i] = the first block of item i, start[n] = the total).  Each block finds its
