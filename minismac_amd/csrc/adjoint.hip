@@ -392,7 +392,8 @@ __global__ void __launch_bounds__(256)
 k_adj_jtv(Dims d, const double *__restrict__ jac, const double *__restrict__ lam,
           double *__restrict__ out)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    const Blk bk = xcd_blk();   // the rows of the 9 offsets in this XCD's L2
+    const int j = bk.x * blockDim.x + threadIdx.x, k = bk.y;
     if (j >= d.jm || k >= d.km) return;
     const long N = d.N, p = P(d, j, k);
     double acc[3] = {0.0, 0.0, 0.0};

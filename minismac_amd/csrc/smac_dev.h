@@ -56,12 +56,16 @@ SMAC_DEV Blk hw_blk() { return Blk{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}
 // L % 8 as its (L / 8)-th, and gets logical block start(L % 8) + L / 8, a bijection onto the
 // grid.  Kernels whose points read neighbouring rows then find the rows shared with the
 // blocks beside them in their own XCD's L2.
-SMAC_DEV Blk xcd_blk()
+SMAC_DEV unsigned xcd_lin()
 {
     constexpr unsigned NX = 8;
     const unsigned nb = gridDim.x * gridDim.y, L = blockIdx.x + gridDim.x * blockIdx.y;
     const unsigned q = nb / NX, r = nb % NX, x = L % NX;
-    const unsigned lg = x * q + (x < r ? x : r) + L / NX;
+    return x * q + (x < r ? x : r) + L / NX;
+}
+SMAC_DEV Blk xcd_blk()
+{
+    const unsigned lg = xcd_lin();
     return Blk{lg % gridDim.x, lg / gridDim.x, gridDim.x, gridDim.y};
 }
 
@@ -70,11 +74,13 @@ SMAC_DEV Blk xcd_blk()
 // other (start[i] = the first block of item i, start[n] = the total).  Each block finds its
 // item and runs the kernel body C::run with the item's arguments and its block index in the
 // item's grid, so every item gets exactly the work (and the arithmetic) of its own launch.
+// Blocks are taken in the XCD-aware order (xcd_lin), so that an item's neighbouring blocks
+// share an L2; no body waits on another block, so the order is free.
 template <class C>
 __global__ void __launch_bounds__(C::THREADS)
 k_multi(const typename C::Args *__restrict__ args, const int *__restrict__ start, int n)
 {
-    const int b = (int)blockIdx.x;
+    const int b = (int)xcd_lin();
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
