@@ -162,8 +162,10 @@ smac_status smac_timings(smac_handle *h, double out[7]);
  * the 3x3 sweeps stream their operands through an LDS ring fed by loader wavefronts),
  * "lr_lines" (1..64, default 64: lines per workgroup of the line-relaxation
  * factorisations), "adj_restart" (4..400, default 100: GMRES restart length of
- * smac_adjoint).  All but the first two are tuning knobs; results do not depend on them
- * beyond the solver tolerances.
+ * smac_adjoint), "adj_reorth" (0..1000, default 100: its Gram-Schmidt projects a vector a
+ * second time when less than this many thousandths of its square norm survive the first
+ * projection; 500 is the DGKS test, 0 never).  All but the first two are tuning knobs;
+ * results do not depend on them beyond the solver tolerances.
  * No counterpart in the reference. */
 smac_status smac_set_option(smac_handle *h, const char *name, int value);
 

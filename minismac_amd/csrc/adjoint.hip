@@ -1016,15 +1016,16 @@ void adj_apply_prec(AdjCtx &c, const double *in, double *out)
                            c.pairs, out, in, out);
 }
 
-// gate = 1 when the projections red[0..n-1] removed more than half of the square norm
-// red[n] of the vector (then it is projected a second time)
-__global__ void k_adj_dgks(int n, const double *__restrict__ red, int *__restrict__ gate)
+// gate = 1 when less than the fraction keep of the square norm red[n] of the vector survives
+// the projections red[0..n-1] (then it is projected a second time; keep = 1/2 is the DGKS
+// test, 0 never projects twice)
+__global__ void k_adj_dgks(int n, const double *__restrict__ red, double keep, int *__restrict__ gate)
 {
     double s2 = 0.0;
     for (int i = threadIdx.x; i < n; i += 64) s2 += red[i] * red[i];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off, 64);
-    if (threadIdx.x == 0) *gate = (red[n] - s2 < 0.5 * red[n]) ? 1 : 0;
+    if (threadIdx.x == 0) *gate = (red[n] - s2 < keep * red[n]) ? 1 : 0;
 }
 
 // out += u * red[0] * binv
@@ -1097,7 +1098,7 @@ int adj_gmres(AdjCtx &c, const double *b, double *x, int maxit, double tol, doub
             // vn's square norm (the DGKS test, decided on the device)
             mdot(c, c.V, j + 2, vn, c.red);
             hipLaunchKernelGGL(k_adj_dgks, dim3(1), dim3(64), 0, c.sm, j + 1,
-                               (const double *)c.red, c.gate);
+                               (const double *)c.red, c.reorth, c.gate);
             hipLaunchKernelGGL(k_adj_maxpy, dim3(nb), dim3(256), 0, c.sm, len, j + 1, 1.0, -1.0,
                                (const double *)c.V, (const double *)c.red, vn,
                                (const int *)nullptr);

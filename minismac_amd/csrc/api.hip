@@ -86,6 +86,7 @@ struct smac_handle {
     int nt = 0;
     double rhs0 = 0.0;            // rhsnorm of pass nt = 1 when this handle made it (else 0)
     int adj_restart = 100;        // GMRES restart length of smac_adjoint
+    int adj_reorth = 100;         // per mille: when its Gram-Schmidt projects a second time
     bool first_sa = true;
     bool keep_lhs = false;
     bool timing = false;
@@ -458,6 +459,11 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
     if (!std::strcmp(name, "lr_lines")) {
         if (value < 1 || value > 64) return fail(SMAC_ERR_ARG, "lr_lines must be in [1, 64]");
         h->lr_lines = value;
+        return SMAC_OK;
+    }
+    if (!std::strcmp(name, "adj_reorth")) {
+        if (value < 0 || value > 1000) return fail(SMAC_ERR_ARG, "adj_reorth must be in [0, 1000]");
+        h->adj_reorth = value;
         return SMAC_OK;
     }
     if (!std::strcmp(name, "adj_restart")) {
@@ -922,6 +928,7 @@ smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const dou
     HIP_TRY(hipMemcpyAsync(&ds, h->st, sizeof ds, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     AdjCtx &c = h->adj;
+    c.reorth = h->adj_reorth * 1e-3;
     c.clift = ds.clift;
     c.cl = ds.cl;
     c.cd = ds.cd;

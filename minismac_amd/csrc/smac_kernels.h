@@ -181,6 +181,8 @@ struct AdjCtx {
     double beta, cdis, vnu, pout, alpha, chord, clift, cl, cd;
     int ivis;
     int m;                       // GMRES restart length
+    double reorth;               // second Gram-Schmidt pass when less than this fraction of
+                                 // the vector's square norm survives the first (0: never)
     const signed char *rtype;    // row type per point
     int npair;
     const int *pairs, *inv;      // wake pairs (0-based) and, per pair, the pair whose partner
