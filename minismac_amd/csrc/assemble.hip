@@ -254,8 +254,10 @@ SMAC_DEV void b_assemble(const Blk &bk, Dims d, DevParams prm, const double *__r
     const double *xix = met, *xiy = met + N, *etx = met + 2 * N, *ety = met + 3 * N;
     double vs[3], vc[9], df[3];
     float vfx[2], udx[2], vdx[2];
-    Line5 L;
-    // xi: assign (ref/conup.f:216-271)
+    Line5 L, Le;
+    // xi: assign (ref/conup.f:216-271); the eta line's loads are issued first, so that their
+    // latency runs under the xi arithmetic
+    load_line(q, etx, ety, dj, N, (long)j, d.jm, k, d.km, Le);
     load_line(q, xix, xiy, dj, N, (long)d.jm * k, 1, j, d.jm, L);
     {
         double bm[9], bp[9];
@@ -276,17 +278,16 @@ SMAC_DEV void b_assemble(const Blk &bk, Dims d, DevParams prm, const double *__r
         }
     }
     // eta: add (ref/conup.f:377-457)
-    load_line(q, etx, ety, dj, N, (long)j, d.jm, k, d.km, L);
     {
         double bm[9], bp[9], bc[9];
-        convect_dir(L, k, d.km, prm.beta, prm.cdis, df, bm, bp, bc);
+        convect_dir(Le, k, d.km, prm.beta, prm.cdis, df, bm, bp, bc);
 #pragma unroll
         for (int e = 0; e < 9; ++e) vc[e] = vc[e] + bc[e];
 #pragma unroll
         for (int m = 0; m < 3; ++m) vs[m] = vs[m] - df[m];
         if (prm.ivis == 1) {
             float vfy[2], udy[2], vdy[2];
-            visc_dir(L, prm.vnu, vfy, udy, vdy);
+            visc_dir(Le, prm.vnu, vfy, udy, vdy);
             // xi viscous terms of the centre and s (ref/visc.f:95-113), then eta (:121-140)
             double a = vfx[0], b = vfx[1];
             vc[4] = vc[4] + a + b; vc[8] = vc[8] + a + b;
