@@ -471,6 +471,12 @@ SMAC_DEV unsigned long long edge_ld(rsrc_t r, unsigned vo, unsigned so)
                               __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, EDGE_POLICY));
 }
 
+// Slots are written one 8-byte word per store.  A buffer store of more than 8 bytes whose
+// offset comes from an SGPR can read its data registers after the next instruction has
+// issued, and the compiler does not keep that instruction off them: with a 16-byte store of
+// words 0-1 followed by the FP32-to-FP64 conversion of word 2 into the same registers, the
+// slot sometimes received word 2 in place of word 0 (seen a few times in a thousand runs of
+// the FP32 sweeps, never in the FP64 ones, whose data registers are not reused there).
 SMAC_DEV void edge_st(double v, rsrc_t r, unsigned vo, unsigned so)
 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, (int)vo, (int)so,
@@ -487,11 +493,6 @@ SMAC_DEV U2 edge_ld2(rsrc_t r, unsigned vo, unsigned so)
                                                                         EDGE_POLICY));
 }
 
-SMAC_DEV void edge_st2(double a, double b, rsrc_t r, unsigned vo, unsigned so)
-{
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, (D2{a, b})), r, (int)vo,
-                                           (int)so, EDGE_POLICY);
-}
 
 // Re-reads one hand-over slot and waits for it.  Written as one asm block so that the
 // compiler's wait-count bookkeeping never sees a load on the (rare) polling path: a load
@@ -926,8 +927,8 @@ struct Band {
         }
         st_vec<NEQ>(o, rO, vo, pstride);   // inactive positions are padding
         if constexpr (NEQ == 3) {
-            edge_st2((double)o[0], (double)o[1], rEout, vout, eout);
-            edge_st((double)o[2], rEout, vout + 16, eout);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) edge_st((double)o[m], rEout, vout + 8 * m, eout);
         } else {
             edge_st((double)o[0], rEout, vout, eout);
         }

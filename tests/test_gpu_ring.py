@@ -73,3 +73,25 @@ def test_batch_register_launch_matches_ring_alone(first_ring):
     finally:
         for s in sol:
             s.close()
+
+
+@pytest.mark.parametrize("ring", [1, 0])
+def test_fp32_sweeps_repeat_bit_identical(ring):
+    """The FP32-LHS band sweeps, run many times over, give the same bits every time.  A
+    16-byte store of two hand-over words with an SGPR offset, whose data registers the next
+    instruction overwrote, used to put the third word in the first now and then (about one
+    run in a hundred of three passes at 945x241, both sweep forms); the slots are now written
+    one 8-byte word per store."""
+    case = naca_case(945, 241, m=0.04, alpha=8.0, reynum=3e6, beta=10.0, impsch=2,
+                     wall_spacing=2e-6, precision=SMAC_FP32_LHS)
+    print(f"repeated FP32-LHS sweeps, lusgs_ring={ring}", flush=True)
+
+    def three_passes():
+        with Solver(case, options={"lusgs_ring": ring}) as s:
+            for _ in range(3):
+                s.run(1)
+            return s.get_state()[0]
+
+    ref = three_passes()
+    differ = [rep for rep in range(150) if not np.array_equal(ref, three_passes())]
+    assert not differ, f"runs {differ} differ from the first"
