@@ -41,8 +41,8 @@ BAND_FWD_BYTES = {"fp64": 33 * 8, "fp32": 24 * 4}
 # run -- rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE at 945x241 (scaled by 2.0 and 1.0, the
 # factors that make the two counters read 256 MiB for a plain 256 MiB device copy), recorded
 # in profiles/lusgs_band_945x241_counters.md
-BAND_FWD_TRAFFIC = {"fp64": (2.0 * 30499.0 + 8223.0) * 1024 / 227745,
-                    "fp32": (2.0 * 14326.0 + 4206.0) * 1024 / 227745}
+BAND_FWD_TRAFFIC = {"fp64": (2.0 * 30497.0 + 8318.0) * 1024 / 227745,
+                    "fp32": (2.0 * 14324.0 + 4300.0) * 1024 / 227745}
 TRAFFIC_SOURCE = "fixed figure from profiles/lusgs_band_945x241_counters.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
 # Algorithmic bytes per grid point of the line-relaxation solve family (two xi and two eta
 # sweeps with their factorisations).  The reference-order chain factors in place and reads
