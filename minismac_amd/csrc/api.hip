@@ -1031,6 +1031,12 @@ struct smac_batch {
     MultiTab<WakeArgs> wake;
     MultiTab<UpdArgs> upd;
     MultiTab<ForceArgs> force;
+    // device-side stop (smac_batch_run_until): per case, whether it still runs, the passes it
+    // made, its reference norm, and its statistics record
+    int *active = nullptr;
+    int *passes = nullptr;
+    double *first = nullptr;
+    DevStats **stp = nullptr;
 };
 
 template <typename A>
@@ -1038,17 +1044,30 @@ static void mt_free(MultiTab<A> &t)
 {
     if (t.args) (void)hipFree(t.args);
     if (t.start) (void)hipFree(t.start);
+    if (t.item_case) (void)hipFree(t.item_case);
     t = MultiTab<A>{};
 }
 
-// a device table of the items with a non-empty grid
+// the items of one multi-case kernel and the batch case each belongs to
 template <typename A>
-static smac_status mt_build(MultiTab<A> &t, const std::vector<A> &items)
+struct Items {
+    std::vector<A> v;
+    std::vector<int> c;
+    void tag(int i) { c.resize(v.size(), i); }   // items added since the last tag are case i's
+};
+
+// a device table of the items with a non-empty grid; `active` is the batch's per-case flags
+template <typename A>
+static smac_status mt_build(MultiTab<A> &t, const Items<A> &items, const int *active)
 {
     mt_free(t);
     std::vector<A> keep;
-    for (const A &a : items)
-        if (a.gx > 0 && a.gy > 0) keep.push_back(a);
+    std::vector<int> kcase;
+    for (size_t i = 0; i < items.v.size(); ++i)
+        if (items.v[i].gx > 0 && items.v[i].gy > 0) {
+            keep.push_back(items.v[i]);
+            kcase.push_back(items.c[i]);
+        }
     std::vector<int> start(keep.size() + 1, 0);
     for (size_t i = 0; i < keep.size(); ++i)
         start[i + 1] = start[i] + (int)(keep[i].gx * keep[i].gy);
@@ -1056,10 +1075,13 @@ static smac_status mt_build(MultiTab<A> &t, const std::vector<A> &items)
     t.nblk = start.back();
     if (t.n == 0) return SMAC_OK;
     if (hipMalloc(&t.args, keep.size() * sizeof(A)) != hipSuccess ||
-        hipMalloc(&t.start, start.size() * sizeof(int)) != hipSuccess)
+        hipMalloc(&t.start, start.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&t.item_case, kcase.size() * sizeof(int)) != hipSuccess)
         return fail(SMAC_ERR_HIP, "hipMalloc failed (batch kernel tables)");
     HIP_TRY(hipMemcpy(t.args, keep.data(), keep.size() * sizeof(A), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.item_case, kcase.data(), kcase.size() * sizeof(int), hipMemcpyHostToDevice));
+    t.active = active;
     return SMAC_OK;
 }
 
@@ -1111,6 +1133,8 @@ static void free_batch(smac_batch *b)
     for (auto &e : b->ev_band)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (void *p : {(void *)b->active, (void *)b->passes, (void *)b->first, (void *)b->stp})
+        if (p) (void)hipFree(p);
     delete b;
 }
 
@@ -1147,86 +1171,93 @@ static smac_status build_multi(smac_batch *b)
     b->multi_ok = true;
     for (smac_handle *h : b->cases) b->multi_ok = b->multi_ok && is_lusgs(h);
     if (!b->multi_ok) return SMAC_OK;
-    std::vector<SaDerivArgs> sd;
-    std::vector<SaAsmArgs> sas;
-    std::vector<SaBcArgs> sbc;
-    std::vector<ShearArgs> ssh, sh[2];
-    std::vector<MidArgs> smid, mid[2];
-    std::vector<UnshearArgs> sun, un[2];
-    std::vector<WakeArgs> swk, wk;
-    std::vector<SaUpdArgs> sup;
-    std::vector<SaTurArgs> stu;
-    std::vector<AsmArgs> as[2];
-    std::vector<BcArgs> bc[2];
-    std::vector<ResidPArgs> rp;
-    std::vector<ResidFArgs> rf;
-    std::vector<UpdArgs> up;
-    std::vector<ForceArgs> fo;
-    for (smac_handle *h : b->cases) {
+    Items<SaDerivArgs> sd;
+    Items<SaAsmArgs> sas;
+    Items<SaBcArgs> sbc;
+    Items<ShearArgs> ssh, sh[2];
+    Items<MidArgs> smid, mid[2];
+    Items<UnshearArgs> sun, un[2];
+    Items<WakeArgs> swk, wk;
+    Items<SaUpdArgs> sup;
+    Items<SaTurArgs> stu;
+    Items<AsmArgs> as[2];
+    Items<BcArgs> bc[2];
+    Items<ResidPArgs> rp;
+    Items<ResidFArgs> rf;
+    Items<UpdArgs> up;
+    Items<ForceArgs> fo;
+    for (size_t ci = 0; ci < b->cases.size(); ++ci) {
+        smac_handle *h = b->cases[ci];
         const Dims &d = h->d;
         const DevParams &dp = h->dp;
         if (has_sa(h)) {
             SaArrays &a = h->sa;
-            sd.push_back(sa_deriv_args(d, h->q, h->met, h->dj, a));
-            sas.push_back(sa_asm_args(d, dp, 0, h->q, h->met, h->dj, a));
-            sa_bc_args(d, (int)h->regs.size(), h->regs.data(), a, sbc);
-            ssh.push_back(shear_args<double>(d, h->lws_sa, a.sf, a.stc, a.sjm, a.sjp, a.skm,
+            sd.v.push_back(sa_deriv_args(d, h->q, h->met, h->dj, a));
+            sas.v.push_back(sa_asm_args(d, dp, 0, h->q, h->met, h->dj, a));
+            sa_bc_args(d, (int)h->regs.size(), h->regs.data(), a, sbc.v);
+            ssh.v.push_back(shear_args<double>(d, h->lws_sa, a.sf, a.stc, a.sjm, a.sjp, a.skm,
                                              a.skp, (double *)nullptr));
-            smid.push_back(mid_args<double, 1>(d, h->lws_sa));
-            sun.push_back(unshear_args(d, h->lws_sa, h->lws_sa.dq, a.ds));
-            swk.push_back(wake_args(d, 1, h->wake, a.anut, a.ds, a.sf));
-            sup.push_back(sa_upd_args(d, dp, h->w, a));
-            stu.push_back(sa_tur_args(h->w, h->st));
+            smid.v.push_back(mid_args<double, 1>(d, h->lws_sa));
+            sun.v.push_back(unshear_args(d, h->lws_sa, h->lws_sa.dq, a.ds));
+            swk.v.push_back(wake_args(d, 1, h->wake, a.anut, a.ds, a.sf));
+            sup.v.push_back(sa_upd_args(d, dp, h->w, a));
+            stu.v.push_back(sa_tur_args(h->w, h->st));
         }
         const int g = h->p.precision == SMAC_FP32_LHS ? 1 : 0;
         if (g) {
             float *fb[5] = {(float *)h->blk[0], (float *)h->blk[1], (float *)h->blk[2],
                             (float *)h->blk[3], (float *)h->blk[4]};
-            as[g].push_back(asm_args<float>(d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2],
+            as[g].v.push_back(asm_args<float>(d, dp, h->q, h->met, h->dj, h->s, fb[0], fb[1], fb[2],
                                             fb[3], fb[4]));
             bc_args<float>(d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y, h->met,
-                           h->q, h->s, fb[0], fb[1], fb[2], fb[3], fb[4], bc[g]);
-            sh[g].push_back(shear_args<float>(d, h->lws, h->s, fb[0], fb[1], fb[2], fb[3], fb[4],
+                           h->q, h->s, fb[0], fb[1], fb[2], fb[3], fb[4], bc[g].v);
+            sh[g].v.push_back(shear_args<float>(d, h->lws, h->s, fb[0], fb[1], fb[2], fb[3], fb[4],
                                               (float *)nullptr));
-            mid[g].push_back(mid_args<float, 3>(d, h->lws));
+            mid[g].v.push_back(mid_args<float, 3>(d, h->lws));
         } else {
             double **db = h->blk;
-            as[g].push_back(asm_args<double>(d, dp, h->q, h->met, h->dj, h->s, db[0], db[1],
+            as[g].v.push_back(asm_args<double>(d, dp, h->q, h->met, h->dj, h->s, db[0], db[1],
                                              db[2], db[3], db[4]));
             bc_args<double>(d, dp, h->regs.data(), (int)h->regs.size(), h->st, h->x, h->y,
-                            h->met, h->q, h->s, db[0], db[1], db[2], db[3], db[4], bc[g]);
-            sh[g].push_back(shear_args<double>(d, h->lws, h->s, db[0], db[1], db[2], db[3], db[4],
+                            h->met, h->q, h->s, db[0], db[1], db[2], db[3], db[4], bc[g].v);
+            sh[g].v.push_back(shear_args<double>(d, h->lws, h->s, db[0], db[1], db[2], db[3], db[4],
                                                (double *)nullptr));
-            mid[g].push_back(mid_args<double, 3>(d, h->lws));
+            mid[g].v.push_back(mid_args<double, 3>(d, h->lws));
         }
-        un[g].push_back(unshear_args(d, h->lws, h->lws.dq, h->dq));
-        rp.push_back(residp_args(d, h->s, h->dj, h->w));
-        rf.push_back(residf_args(d, 2, dp.beta, h->s, h->dj, h->w, h->st));   // nt > 1
-        wk.push_back(wake_args(d, 3, h->wake, h->q, h->dq, h->s));
-        up.push_back(upd_args(d, h->q, h->dq, h->w));
-        fo.push_back(force_args(d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->w, h->st));
+        un[g].v.push_back(unshear_args(d, h->lws, h->lws.dq, h->dq));
+        rp.v.push_back(residp_args(d, h->s, h->dj, h->w));
+        rf.v.push_back(residf_args(d, 2, dp.beta, h->s, h->dj, h->w, h->st));   // nt > 1
+        wk.v.push_back(wake_args(d, 3, h->wake, h->q, h->dq, h->s));
+        up.v.push_back(upd_args(d, h->q, h->dq, h->w));
+        fo.v.push_back(force_args(d, dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->w, h->st));
+        const int i = (int)ci;
+        sd.tag(i); sas.tag(i); sbc.tag(i); ssh.tag(i); smid.tag(i); sun.tag(i); swk.tag(i);
+        sup.tag(i); stu.tag(i); rp.tag(i); rf.tag(i); wk.tag(i); up.tag(i); fo.tag(i);
+        for (int gg = 0; gg < 2; ++gg) {
+            as[gg].tag(i); bc[gg].tag(i); sh[gg].tag(i); mid[gg].tag(i); un[gg].tag(i);
+        }
     }
-    B_TRY(mt_build(b->sa_deriv, sd));
-    B_TRY(mt_build(b->sa_asm, sas));
-    B_TRY(mt_build(b->sa_bc, sbc));
-    B_TRY(mt_build(b->sa_shear, ssh));
-    B_TRY(mt_build(b->sa_mid, smid));
-    B_TRY(mt_build(b->sa_unshear, sun));
-    B_TRY(mt_build(b->sa_wake, swk));
-    B_TRY(mt_build(b->sa_upd, sup));
-    B_TRY(mt_build(b->sa_tur, stu));
+    B_TRY(mt_build(b->sa_deriv, sd, b->active));
+    B_TRY(mt_build(b->sa_asm, sas, b->active));
+    B_TRY(mt_build(b->sa_bc, sbc, b->active));
+    B_TRY(mt_build(b->sa_shear, ssh, b->active));
+    B_TRY(mt_build(b->sa_mid, smid, b->active));
+    B_TRY(mt_build(b->sa_unshear, sun, b->active));
+    B_TRY(mt_build(b->sa_wake, swk, b->active));
+    B_TRY(mt_build(b->sa_upd, sup, b->active));
+    B_TRY(mt_build(b->sa_tur, stu, b->active));
     for (int g = 0; g < 2; ++g) {
-        B_TRY(mt_build(b->asm_[g], as[g]));
-        B_TRY(mt_build(b->bc[g], bc[g]));
-        B_TRY(mt_build(b->shear[g], sh[g]));
-        B_TRY(mt_build(b->mid[g], mid[g]));
-        B_TRY(mt_build(b->unshear[g], un[g]));
+        B_TRY(mt_build(b->asm_[g], as[g], b->active));
+        B_TRY(mt_build(b->bc[g], bc[g], b->active));
+        B_TRY(mt_build(b->shear[g], sh[g], b->active));
+        B_TRY(mt_build(b->mid[g], mid[g], b->active));
+        B_TRY(mt_build(b->unshear[g], un[g], b->active));
     }
-    B_TRY(mt_build(b->resp, rp));
-    B_TRY(mt_build(b->resf, rf));
-    B_TRY(mt_build(b->wake, wk));
-    B_TRY(mt_build(b->upd, up));
-    B_TRY(mt_build(b->force, fo));
+    B_TRY(mt_build(b->resp, rp, b->active));
+    B_TRY(mt_build(b->resf, rf, b->active));
+    B_TRY(mt_build(b->wake, wk, b->active));
+    B_TRY(mt_build(b->upd, up, b->active));
+    B_TRY(mt_build(b->force, fo, b->active));
     return SMAC_OK;
 }
 
@@ -1249,13 +1280,13 @@ static smac_status build_tables(smac_batch *b)
         w.ev_fwd[0] = w.ev_fwd[1] = nullptr;
         const int g = h->p.precision == SMAC_FP32_LHS ? 1 : 0;
         b->mean[g].members.push_back((int)i);
-        jm[g].push_back(BandJob{h->d, w});
+        jm[g].push_back(BandJob{h->d, w, (int)i});
         if (!pf_m[g]) pf_m[g] = w.ring ? -1 : w.pf3;
         if (has_sa(h)) {
             LusgsWs ws = h->lws_sa;
             ws.ev_fwd[0] = ws.ev_fwd[1] = nullptr;
             b->sa.members.push_back((int)i);
-            jsa.push_back(BandJob{h->d, ws});
+            jsa.push_back(BandJob{h->d, ws, (int)i});
             if (!pf_sa) pf_sa = ws.pf1;
         }
     }
@@ -1318,9 +1349,9 @@ static smac_status enqueue_batch_pass_multi(smac_batch *b, bool time_band)
     multi_shear<double, 1>(ss, b->sa_shear);
     BandGroup &gs = b->sa;
     if (gs.nblk) {
-        lusgs_band_multi<double, 1>(ss, gs.jobs, gs.map, gs.nblk, gs.pf, true);
+        lusgs_band_multi<double, 1>(ss, gs.jobs, gs.map, gs.nblk, gs.pf, true, b->active);
         multi_mid<double, 1>(ss, b->sa_mid);
-        lusgs_band_multi<double, 1>(ss, gs.jobs, gs.map, gs.nblk, gs.pf, false);
+        lusgs_band_multi<double, 1>(ss, gs.jobs, gs.map, gs.nblk, gs.pf, false, b->active);
     }
     multi_unshear<double, 1>(ss, b->sa_unshear);
     multi_wake(ss, b->sa_wake);
@@ -1342,15 +1373,15 @@ static smac_status enqueue_batch_pass_multi(smac_batch *b, bool time_band)
         const bool rec = time_band && g == b->band_group;
         if (rec) HIP_TRY(hipEventRecord(b->ev_band[0], sm));
         if (g == 0)
-            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, b->active);
         else
-            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, b->active);
         if (rec) HIP_TRY(hipEventRecord(b->ev_band[1], sm));
         if (g == 0) {
             multi_mid<double, 3>(sm, b->mid[0]);
-            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, b->active);
         } else {
-            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, b->active);
         }
     }
     multi_unshear<double, 3>(sm, b->unshear[0]);
@@ -1400,14 +1431,14 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
     B_TRY(join_lanes(b));
     BandGroup &gs = b->sa;
     if (gs.nblk) {
-        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true);
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true, nullptr);
         B_TRY(fork_lanes(b));
         for (size_t n = 0; n < gs.members.size(); ++n) {
             smac_handle *h = b->cases[gs.members[n]];
             lusgs_mid<double, 1>(lane_of(gs.members[n]), h->d, h->lws_sa);
         }
         B_TRY(join_lanes(b));
-        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false);
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false, nullptr);
     }
     // SA after its sweeps; mean flow up to its band sweeps (line relaxation: the whole solve)
     B_TRY(fork_lanes(b));
@@ -1451,9 +1482,9 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
         const bool rec = time_band && g == b->band_group;
         if (rec) HIP_TRY(hipEventRecord(b->ev_band[0], sm));
         if (g == 0)
-            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, nullptr);
         else
-            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true);
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, nullptr);
         if (rec) HIP_TRY(hipEventRecord(b->ev_band[1], sm));
         B_TRY(fork_lanes(b));
         for (int i : gm.members) {
@@ -1464,9 +1495,9 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
         }
         B_TRY(join_lanes(b));
         if (g == 0)
-            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, nullptr);
         else
-            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false);
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, nullptr);
     }
     B_TRY(fork_lanes(b));
     for (int i = 0; i < nc; ++i) {
@@ -1522,6 +1553,20 @@ smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out)
         free_batch(b);
         return fail(SMAC_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
+    {
+        std::vector<int> ones(n, 1);
+        std::vector<DevStats *> stp(n);
+        for (int i = 0; i < n; ++i) stp[i] = cases[i]->st;
+        if (hipMalloc(&b->active, n * sizeof(int)) != hipSuccess ||
+            hipMalloc(&b->passes, n * sizeof(int)) != hipSuccess ||
+            hipMalloc(&b->first, n * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b->stp, n * sizeof(DevStats *)) != hipSuccess ||
+            hipMemcpy(b->active, ones.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->stp, stp.data(), n * sizeof(DevStats *), hipMemcpyHostToDevice) != hipSuccess) {
+            free_batch(b);
+            return fail(SMAC_ERR_HIP, "hipMalloc / hipMemcpy failed (batch flags)");
+        }
+    }
     smac_status rc = build_tables(b);
     if (rc) {
         free_batch(b);
@@ -1541,6 +1586,10 @@ smac_status smac_batch_run(smac_batch *b, int npass)
         HIP_TRY(hipStreamSynchronize(h->stream_sa));
     }
     B_TRY(check_tables(b));
+    {
+        std::vector<int> ones(b->cases.size(), 1);   // every case runs every pass
+        HIP_TRY(hipMemcpy(b->active, ones.data(), ones.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     // the mean-flow group with the most points is timed in the last pass
     double pts[2] = {0.0, 0.0};
     for (int g = 0; g < 2; ++g)
@@ -1564,6 +1613,68 @@ smac_status smac_batch_run(smac_batch *b, int npass)
         if (s.nonfinite)
             return fail(SMAC_ERR_NONFINITE, "batch case " + std::to_string(i) +
                                                 ": non-finite at nt=" + std::to_string(h->nt));
+    }
+    return SMAC_OK;
+}
+
+// Runs up to max_pass passes; each case stops on the device after the pass whose residual
+// norm is at most drop times its first pass's (smac_run's criterion), so the host enqueues
+// passes without reading any case's statistics.  It looks at the cases' flags every 16
+// passes and returns once every case has stopped.
+smac_status smac_batch_run_until(smac_batch *b, int max_pass, double drop, int *passes_out)
+{
+    if (!b) return fail(SMAC_ERR_ARG, "batch is NULL");
+    if (max_pass < 0) return fail(SMAC_ERR_ARG, "max_pass < 0");
+    if (!(drop >= 0.0)) return fail(SMAC_ERR_ARG, "drop must be >= 0");
+    for (smac_handle *h : b->cases) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream_sa));
+    }
+    B_TRY(check_tables(b));
+    if (!b->multi_ok)
+        return fail(SMAC_ERR_ARG, "the device-side stop needs every case of the batch on LU-SGS");
+    const int n = (int)b->cases.size();
+    std::vector<int> flag(n, 1), cnt(n, 0), nt0(n);
+    std::vector<double> first(n);
+    for (int i = 0; i < n; ++i) {
+        nt0[i] = b->cases[i]->nt;
+        first[i] = b->cases[i]->rhs0 > 0.0 ? b->cases[i]->rhs0 : 0.0;   // 0: this run's first
+    }
+    HIP_TRY(hipMemcpy(b->active, flag.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->passes, cnt.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->first, first.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    for (int it = 0; it < max_pass; ++it) {
+        B_TRY(enqueue_batch_pass(b, false));
+        launch_batch_gate(b->stream, n, b->stp, b->first, drop, b->active, b->passes);
+        HIP_TRY(hipGetLastError());
+        if ((it + 1) % 16 == 0 && it + 1 < max_pass) {
+            HIP_TRY(hipMemcpy(flag.data(), b->active, n * sizeof(int), hipMemcpyDeviceToHost));
+            bool any = false;
+            for (int v : flag) any = any || v != 0;
+            if (!any) break;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(cnt.data(), b->passes, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(first.data(), b->first, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        smac_handle *h = b->cases[i];
+        h->nt = nt0[i] + cnt[i];   // the host counted every enqueued pass
+        if (nt0[i] == 0 && cnt[i] > 0) h->rhs0 = first[i];
+        if (passes_out) passes_out[i] = cnt[i];
+    }
+    b->band_ms = 0.0;
+    b->band_pts = 0.0;
+    b->band_group = -1;
+    for (int i = 0; i < n; ++i) {
+        DevStats s;
+        HIP_TRY(hipMemcpy(&s, b->cases[i]->st, sizeof s, hipMemcpyDeviceToHost));
+        if (s.stall)
+            return fail(SMAC_ERR_STALL, "batch case " + std::to_string(i) +
+                                            ": LU-SGS band hand-over timed out");
+        if (s.nonfinite)
+            return fail(SMAC_ERR_NONFINITE, "batch case " + std::to_string(i) +
+                                                ": non-finite at nt=" + std::to_string(b->cases[i]->nt));
     }
     return SMAC_OK;
 }

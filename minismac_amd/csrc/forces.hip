@@ -226,16 +226,41 @@ struct CForce {
     }
 };
 
+// Device-side stop of a batch: after each pass, every case still running counts the pass
+// and stops once its residual norm has dropped to `drop` times its reference norm (the norm
+// of its first pass, as smac_run's stop criterion), or once a pass of it went non-finite or
+// stalled.  A stopped case's items in later passes return at once (k_multi, the multi-case
+// band sweeps), so its state stays that of the pass that met the criterion.
+__global__ void k_batch_gate(int n, DevStats *const *__restrict__ st, double *__restrict__ first,
+                             double drop, int *__restrict__ active, int *__restrict__ passes)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n || active[i] == 0) return;
+    passes[i] += 1;
+    const DevStats *s = st[i];
+    const double r = s->rhsnorm;
+    if (!(first[i] > 0.0)) first[i] = r;
+    if (s->nonfinite || s->stall || (drop > 0.0 && r <= drop * first[i])) active[i] = 0;
+}
+
+void launch_batch_gate(hipStream_t sm, int n, DevStats *const *st, double *first, double drop,
+                       int *active, int *passes)
+{
+    hipLaunchKernelGGL(k_batch_gate, dim3((n + 63) / 64), dim3(64), 0, sm, n, st, first, drop,
+                       active, passes);
+}
+
 void multi_update(hipStream_t sm, const MultiTab<UpdArgs> &t)
 {
     if (t.n > 0 && t.nblk > 0)
-        hipLaunchKernelGGL(k_multi<CUpdate>, dim3(t.nblk), dim3(256), 0, sm, t.args, t.start, t.n);
+        hipLaunchKernelGGL(k_multi<CUpdate>, dim3(t.nblk), dim3(256), 0, sm, t.args, t.start, t.n,
+                           t.item_case, t.active);
 }
 void multi_force(hipStream_t sm, const MultiTab<ForceArgs> &t)
 {
     if (t.n > 0 && t.nblk > 0)
         hipLaunchKernelGGL(k_multi<CForce>, dim3(t.nblk), dim3(FORCE_THREADS), 0, sm, t.args,
-                           t.start, t.n);
+                           t.start, t.n, t.item_case, t.active);
 }
 
 }  // namespace smac

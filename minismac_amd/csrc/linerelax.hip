@@ -380,7 +380,8 @@ struct CWake {
 void multi_wake(hipStream_t sm, const MultiTab<WakeArgs> &t)
 {
     if (t.n > 0 && t.nblk > 0)
-        hipLaunchKernelGGL(k_multi<CWake>, dim3(t.nblk), dim3(64), 0, sm, t.args, t.start, t.n);
+        hipLaunchKernelGGL(k_multi<CWake>, dim3(t.nblk), dim3(64), 0, sm, t.args, t.start, t.n,
+                           t.item_case, t.active);
 }
 
 void launch_wake(hipStream_t sm, const Dims &d, int neqs, const WakePairs &w, const double *sq,

@@ -1146,9 +1146,12 @@ k_band(Dims d, LusgsWs w)
 // workgroup dispatched earlier.
 template <int NEQ, int PF, bool FWD, typename T>
 __global__ void __launch_bounds__(BAND_LANES)
-k_band_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map)
+k_band_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map,
+             const int *__restrict__ active)
 {
     const int2 m = map[blockIdx.x];
+    // a stopped case: all of its bands leave, so none waits on another
+    if (active && active[jobs[m.x].bcase] == 0) return;
     const Dims d = jobs[m.x].d;
     const LusgsWs w = jobs[m.x].w;
     band_run<NEQ, PF, FWD, T>(d, w, m.y);
@@ -1164,9 +1167,11 @@ k_band_ring(Dims d, LusgsWs w)
 
 template <bool FWD, typename T>
 __global__ void __launch_bounds__(BAND_LANES * (1 + RING_LOADERS))
-k_band_ring_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map)
+k_band_ring_multi(const BandJob *__restrict__ jobs, const int2 *__restrict__ map,
+                  const int *__restrict__ active)
 {
     const int2 m = map[blockIdx.x];
+    if (active && active[jobs[m.x].bcase] == 0) return;
     const Dims d = jobs[m.x].d;
     const LusgsWs w = jobs[m.x].w;
     band_run<3, RING_PF, FWD, T, true>(d, w, m.y);
@@ -1252,19 +1257,21 @@ void lusgs_band(hipStream_t sm, const Dims &d, const LusgsWs &w, bool fwd)
 
 template <typename LT, int NEQ>
 void lusgs_band_multi(hipStream_t sm, const BandJob *jobs, const int2 *map, int nblk, int pf,
-                      bool fwd)
+                      bool fwd, const int *active)
 {
     if (nblk <= 0) return;
     if (NEQ == 3 && pf == 0) {   // pf 0 selects the LDS-ring sweeps
         const dim3 blk(BAND_LANES * (1 + RING_LOADERS));
         if (fwd)
-            hipLaunchKernelGGL((k_band_ring_multi<true, LT>), dim3(nblk), blk, 0, sm, jobs, map);
+            hipLaunchKernelGGL((k_band_ring_multi<true, LT>), dim3(nblk), blk, 0, sm, jobs, map,
+                               active);
         else
-            hipLaunchKernelGGL((k_band_ring_multi<false, LT>), dim3(nblk), blk, 0, sm, jobs, map);
+            hipLaunchKernelGGL((k_band_ring_multi<false, LT>), dim3(nblk), blk, 0, sm, jobs, map,
+                               active);
         return;
     }
-#define MUL_F(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, true, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map)
-#define MUL_B(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, false, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map)
+#define MUL_F(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, true, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map, active)
+#define MUL_B(PFV) hipLaunchKernelGGL((k_band_multi<NEQ, PFV, false, LT>), dim3(nblk), dim3(BAND_LANES), 0, sm, jobs, map, active)
     if (fwd)
         BAND_SWITCH(pf, MUL_F);
     else
@@ -1350,7 +1357,8 @@ template <class C>
 static void multi_launch(hipStream_t sm, const MultiTab<typename C::Args> &t)
 {
     if (t.n > 0 && t.nblk > 0)
-        hipLaunchKernelGGL(k_multi<C>, dim3(t.nblk), dim3(C::THREADS), 0, sm, t.args, t.start, t.n);
+        hipLaunchKernelGGL(k_multi<C>, dim3(t.nblk), dim3(C::THREADS), 0, sm, t.args, t.start, t.n,
+                           t.item_case, t.active);
 }
 
 template <typename LT, int NEQ> void multi_shear(hipStream_t sm, const MultiTab<ShearArgs> &t)
@@ -1421,7 +1429,7 @@ void launch_lusgs1(hipStream_t sm, const Dims &d, const LusgsWs &w, const WakePa
                                      LT *);                                                    \
     template void lusgs_band<LT, NEQ>(hipStream_t, const Dims &, const LusgsWs &, bool);       \
     template void lusgs_band_multi<LT, NEQ>(hipStream_t, const BandJob *, const int2 *, int,   \
-                                            int, bool);                                        \
+                                            int, bool, const int *);                           \
     template void lusgs_mid<LT, NEQ>(hipStream_t, const Dims &, const LusgsWs &);              \
     template void lusgs_post<LT, NEQ>(hipStream_t, const Dims &, const LusgsWs &,              \
                                       const WakePairs &, const double *, double *, double *,   \

@@ -558,7 +558,8 @@ template <class C>
 static void multi_launch(hipStream_t sm, const MultiTab<typename C::Args> &t)
 {
     if (t.n > 0 && t.nblk > 0)
-        hipLaunchKernelGGL(k_multi<C>, dim3(t.nblk), dim3(C::THREADS), 0, sm, t.args, t.start, t.n);
+        hipLaunchKernelGGL(k_multi<C>, dim3(t.nblk), dim3(C::THREADS), 0, sm, t.args, t.start, t.n,
+                           t.item_case, t.active);
 }
 
 void multi_sa_deriv(hipStream_t sm, const MultiTab<SaDerivArgs> &t) { multi_launch<CSaDeriv>(sm, t); }

@@ -78,7 +78,8 @@ SMAC_DEV Blk xcd_blk()
 // share an L2; no body waits on another block, so the order is free.
 template <class C>
 __global__ void __launch_bounds__(C::THREADS)
-k_multi(const typename C::Args *__restrict__ args, const int *__restrict__ start, int n)
+k_multi(const typename C::Args *__restrict__ args, const int *__restrict__ start, int n,
+        const int *__restrict__ item_case, const int *__restrict__ active)
 {
     const int b = (int)xcd_lin();
     int lo = 0, hi = n - 1;
@@ -87,6 +88,7 @@ k_multi(const typename C::Args *__restrict__ args, const int *__restrict__ start
         if (start[mid] <= b) lo = mid;
         else hi = mid - 1;
     }
+    if (active[item_case[lo]] == 0) return;   // the case has stopped (device-side test)
     const typename C::Args &a = args[lo];
     const unsigned lb = (unsigned)(b - start[lo]);
     C::run(Blk{lb % a.gx, lb / a.gx, a.gx, a.gy}, a);

@@ -107,6 +107,7 @@ struct LusgsWs {
 struct BandJob {
     Dims d;
     LusgsWs w;
+    int bcase;   // the case's index in its batch (selects its entry of the batch's active flags)
 };
 // the solve in phases, for callers that put the band sweeps of several cases in one launch:
 // pre (transpose and factor), band (forward / backward), mid, post (back to the natural
@@ -119,7 +120,7 @@ void lusgs_band(hipStream_t, const Dims &, const LusgsWs &, bool fwd);
 // jobs and map in device memory; map[i] = (job, band in dispatch order), nblk entries
 template <typename LT, int NEQ>
 void lusgs_band_multi(hipStream_t, const BandJob *jobs, const int2 *map, int nblk, int pf,
-                      bool fwd);
+                      bool fwd, const int *active);
 template <typename LT, int NEQ>
 void lusgs_mid(hipStream_t, const Dims &, const LusgsWs &);
 template <typename LT, int NEQ>
@@ -164,6 +165,8 @@ void launch_update_force(hipStream_t, const Dims &, const DevParams &, const Wal
 // the two halves of launch_update_force: q += dq (raising the non-finite flag), then the
 // wall forces, CL/CD and clift (clearing the flag into st->nonfinite)
 void launch_update(hipStream_t, const Dims &, double *q, const double *dq, Scratch &);
+void launch_batch_gate(hipStream_t, int n, DevStats *const *st, double *first, double drop,
+                       int *active, int *passes);
 void launch_force(hipStream_t, const Dims &, const DevParams &, const Walls &, const double *x,
                   const double *y, const double *met, const double *dj, const double *q,
                   Scratch &, DevStats *);
@@ -214,6 +217,8 @@ template <typename A>
 struct MultiTab {
     A *args = nullptr;
     int *start = nullptr;
+    int *item_case = nullptr;     // device: the batch case of each item
+    const int *active = nullptr;  // device: the batch's per-case flags; 0 = item skipped
     int n = 0, nblk = 0;
 };
 

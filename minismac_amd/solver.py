@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsmac_amd
 EXPORTS = [
     "smac_create", "smac_destroy", "smac_set_state", "smac_get_state", "smac_step", "smac_run",
     "smac_get_array", "smac_timings", "smac_set_option", "smac_lrelax", "smac_lusgs",
-    "smac_get_stats", "smac_batch_create", "smac_batch_run", "smac_batch_timings",
+    "smac_get_stats", "smac_batch_create", "smac_batch_run", "smac_batch_run_until",
+    "smac_batch_timings",
     "smac_batch_destroy", "smac_lusgs_noalloc", "smac_adjoint", "smac_adjoint_apply",
     "smac_last_error", "smac_version",
 ]
@@ -61,6 +62,7 @@ def lib():
         L.smac_get_stats.argtypes = [C.c_void_p, P(SmacIterStats)]
         L.smac_batch_create.argtypes = [P(C.c_void_p), C.c_int, P(C.c_void_p)]
         L.smac_batch_run.argtypes = [C.c_void_p, C.c_int]
+        L.smac_batch_run_until.argtypes = [C.c_void_p, C.c_int, C.c_double, P(C.c_int)]
         L.smac_batch_timings.argtypes = [C.c_void_p, D]
         L.smac_batch_destroy.argtypes = [C.c_void_p]
         L.smac_adjoint.argtypes = [C.c_void_p, C.c_int, D, D, C.c_int, C.c_double, D, D,
@@ -215,6 +217,15 @@ class Batch:
 
     def run(self, npass: int) -> None:
         _check(lib().smac_batch_run(self.b, int(npass)), "smac_batch_run")
+
+    def run_until(self, max_pass: int, drop: float) -> list:
+        """Up to max_pass passes, each case stopping on the device after the pass that drops
+        its residual norm to `drop` times its first pass's (smac_run's criterion); returns
+        the passes each case made."""
+        out = (C.c_int * len(self.solvers))()
+        _check(lib().smac_batch_run_until(self.b, int(max_pass), float(drop), out),
+               "smac_batch_run_until")
+        return list(out)
 
     def timings(self) -> dict:
         """The forward mean-flow band launch of the last pass: device ms, grid points it

@@ -165,3 +165,43 @@ def test_batch_workload_generator_cases_step():
     finally:
         for s in sol:
             s.close()
+
+
+def test_device_side_stop_matches_smac_run():
+    """smac_batch_run_until: each case stops on the device after the pass that drops its
+    residual norm to `drop` times its first pass's, so its pass count and state equal
+    smac_run(max_pass, drop) on it alone -- cases that stop early, one that runs every pass,
+    both precisions and two sizes.  A second call continues from the stopped states with the
+    same reference norms, and a plain run afterwards advances every case again."""
+    def steady(jm, km, **kw):
+        return naca_case(jm, km, reynum=1000.0, beta=10.0, impsch=2, **kw)
+    cases = [steady(119, 31, alpha=4.0, dtau=10.0),
+             steady(119, 31, m=0.02, alpha=8.0, dtau=5.0),
+             config_case(0, impsch=2),
+             steady(119, 31, alpha=4.0, dtau=10.0, precision=SMAC_FP32_LHS),
+             steady(239, 61, alpha=2.0, dtau=10.0)]
+    max_pass, drop = 150, 5e-4
+    print(f"device-side stop: {len(cases)} cases, max_pass={max_pass}, drop={drop}", flush=True)
+    alone = [Solver(c) for c in cases]
+    sol = [Solver(c) for c in cases]
+    try:
+        done = [s.run(max_pass, drop)[1] for s in alone]
+        with Batch(sol) as b:
+            passes = b.run_until(max_pass, drop)
+            assert passes == done, (passes, done)
+            assert max(done) == max_pass and min(done) < max_pass, done
+            for s0, s1 in zip(alone, sol):
+                assert all(np.array_equal(a, c) for a, c in zip(s0.get_state(), s1.get_state()))
+                assert s0.stats()["rhsnorm"] == s1.stats()["rhsnorm"]
+            # again from the stopped states: the same reference norms as smac_run keeps
+            done2 = [s.run(max_pass, drop)[1] for s in alone]
+            assert b.run_until(max_pass, drop) == done2
+            # a plain batch run advances every case
+            for s in alone:
+                s.run(3)
+            b.run(3)
+        for s0, s1 in zip(alone, sol):
+            assert all(np.array_equal(a, c) for a, c in zip(s0.get_state(), s1.get_state()))
+    finally:
+        for s in alone + sol:
+            s.close()
