@@ -258,6 +258,48 @@ def batched_run(n: int, steps: int, warmup: int, seed: int = 0) -> dict:
                     "smac_run"}
 
 
+def batched_drop_run(n: int, max_pass: int, drop: float, seed: int = 1) -> dict:
+    """n seeded steady cases (the batched workload's generator at Re = 1000, dtau = 10,
+    FP32-LHS LU-SGS) run to a residual drop: together with the stop test on the device
+    (smac_batch_run_until, each case stopping after its own pass that meets the drop), and one
+    after another with smac_run's host-side test; the pass counts and states must agree."""
+    import numpy as np
+    from minismac_amd._abi import SMAC_FP32_LHS
+    from minismac_amd.case import batch_cases
+    from minismac_amd.solver import Batch, Solver
+    cases = batch_cases(n, seed=seed, impsch=2, precision=SMAC_FP32_LHS, reynum=1000.0,
+                        dtau=10.0)
+    sol = [Solver(c) for c in cases]
+    alone = []
+    try:
+        with Batch(sol) as b:
+            t0 = time.perf_counter()
+            passes = b.run_until(max_pass, drop)
+            wall = time.perf_counter() - t0
+        alone = [Solver(c) for c in cases]
+        t0 = time.perf_counter()
+        done = [s.run(max_pass, drop)[1] for s in alone]
+        wall_seq = time.perf_counter() - t0
+        same = passes == done and all(
+            all(np.array_equal(a, c) for a, c in zip(s0.get_state(), s1.get_state()))
+            for s0, s1 in zip(alone, sol))
+    finally:
+        for s in sol + alone:
+            s.close()
+    cell_passes = sum(c.npts * p for c, p in zip(cases, passes))
+    stopped = sum(1 for p in passes if p < max_pass)
+    return {"value": round(cell_passes / wall / 1e6, 4), "unit": "Mcell-iters/s",
+            "cases": n, "seed": seed, "drop": drop, "max_pass": max_pass,
+            "stopped_early": stopped, "passes_min_median_max": [
+                int(min(passes)), int(sorted(passes)[len(passes) // 2]), int(max(passes))],
+            "seconds": round(wall, 3), "sequential_seconds": round(wall_seq, 3),
+            "speedup_vs_sequential": round(wall_seq / wall, 4),
+            "same_as_sequential": same, "precision": "fp32-lhs",
+            "note": "value counts the passes each case made (a stopped case's later passes do "
+                    "nothing); the host reads no statistics between passes, only the cases' "
+                    "flags every 16 passes"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +312,8 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-LHS measurement")
     ap.add_argument("--batch-cases", type=int, default=64,
                     help="cases in the batched measurement (0 skips it)")
+    ap.add_argument("--drop-cases", type=int, default=32,
+                    help="cases in the batched run to a residual drop (0 skips it)")
     ap.add_argument("--no-lr", action="store_true", help="skip the line-relaxation forms")
     ap.add_argument("--lr-forms", default="",
                     help="comma list of the line-relaxation forms to time (default: all of "
@@ -380,6 +424,10 @@ def main() -> int:
     if args.batch_cases > 0:
         print(f"bench: batched workload, {args.batch_cases} cases", file=sys.stderr, flush=True)
         line["batched"] = batched_run(args.batch_cases, args.steps, args.warmup)
+    if args.drop_cases > 0:
+        print(f"bench: batched run to a drop, {args.drop_cases} cases", file=sys.stderr,
+              flush=True)
+        line["batched_to_drop"] = batched_drop_run(args.drop_cases, 600, 5e-4)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_iters)
     print(json.dumps(line), flush=True)

@@ -123,6 +123,15 @@ module smac_amd
       integer(c_int), value :: npass
     end function
 
+    integer(c_int) function smac_batch_run_until(b, max_pass, drop, passes) &
+        bind(C, name="smac_batch_run_until")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: max_pass
+      real(c_double), value :: drop
+      integer(c_int) :: passes(*)
+    end function
+
     integer(c_int) function smac_batch_timings(b, out) bind(C, name="smac_batch_timings")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: b
