@@ -117,7 +117,8 @@ smac_status smac_step(smac_handle *h, smac_iter_stats *st);
  * round trips and the stats are read once, after the last.  *done = passes made. */
 smac_status smac_run(smac_handle *h, int niter, double drop, smac_iter_stats *last, int *done);
 
-/* Stats of the last pass made on h (by smac_step, smac_run or smac_batch_run). */
+/* Stats of the last pass made on h (by smac_step, smac_run, smac_batch_run or
+ * smac_batch_run_until). */
 smac_status smac_get_stats(smac_handle *h, smac_iter_stats *st);
 
 /* ---- Batches of cases (no counterpart in the reference: its cases are separate runs) ----

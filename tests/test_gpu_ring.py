@@ -7,7 +7,7 @@ import pytest
 
 from minismac_amd._abi import SMAC_FP32_LHS
 from minismac_amd.case import config_case, naca_case
-from minismac_amd.solver import Batch, Solver
+from minismac_amd.solver import Batch, SmacError, Solver
 
 pytestmark = pytest.mark.gpu
 
@@ -88,8 +88,17 @@ def test_fp32_sweeps_repeat_bit_identical(ring):
 
     def three_passes():
         with Solver(case, options={"lusgs_ring": ring}) as s:
-            for _ in range(3):
-                s.run(1)
+            for p in range(3):
+                try:
+                    s.run(1)
+                except SmacError as e:   # say where the values went non-finite
+                    where = {}
+                    for name in ("q", "dq", "s"):
+                        j, k, m = np.nonzero(~np.isfinite(s.array(name)))   # (jmax, kmax, 3)
+                        where[name] = (len(j), sorted(set(k.tolist()))[:8],
+                                       list(zip(j[:4].tolist(), k[:4].tolist(), m[:4].tolist())))
+                    raise AssertionError(f"pass {p + 1}: {e}; non-finite (count, rows, "
+                                         f"first (j,k,m)): {where}") from e
             return s.get_state()[0]
 
     ref = three_passes()
