@@ -205,3 +205,18 @@ def test_device_side_stop_matches_smac_run():
     finally:
         for s in alone + sol:
             s.close()
+
+
+def test_device_side_stop_needs_lusgs_cases():
+    """The device-side stop works on the multi-case passes, which need every case on LU-SGS: a
+    batch with a line-relaxation case is refused, and its cases are left untouched."""
+    cases = [config_case(0, impsch=2), config_case(0, impsch=1)]
+    sol = [Solver(c) for c in cases]
+    try:
+        with Batch(sol) as b:
+            with pytest.raises(SmacError):
+                b.run_until(10, 1e-3)
+        assert all(s.stats()["nt"] == 0 for s in sol)
+    finally:
+        for s in sol:
+            s.close()
