@@ -136,14 +136,14 @@ smac_status smac_batch_create(smac_handle **cases, int n, smac_batch **out);
 /* npass passes of every case; returns after the device has finished them, having checked
  * each case's non-finite and stall flags. */
 smac_status smac_batch_run(smac_batch *b, int npass);
-/* Up to max_pass passes with the stop test on the device, per case: a case stops after the
- * pass whose ||s||_2 is at most drop times its first pass's, the criterion of smac_run and of
- * the reference's time loop (ref/smac2d.f:1376-1431), or after a non-finite or stalled pass;
- * its later passes do nothing, so its state and pass count equal smac_run(h, max_pass, drop)
- * on it alone.  The host enqueues the passes without reading statistics (it looks at the
- * cases' flags every 16 passes and returns once all have stopped).  passes_out (n ints, may
- * be NULL) receives each case's passes.  Every case must use LU-SGS (impsch 2); drop 0 runs
- * every case max_pass passes. */
+/* Up to max_pass passes with a stop test per case: a case stops after the pass whose ||s||_2
+ * is at most drop times its first pass's, the criterion of smac_run and of the reference's
+ * time loop (ref/smac2d.f:1376-1431), or after a non-finite or stalled pass; it gets no
+ * further passes, so its state and pass count equal smac_run(h, max_pass, drop) on it alone.
+ * With every case on LU-SGS (impsch 2) the test runs on the device and the host enqueues the
+ * passes without reading statistics (it looks at the cases' flags every 16 passes and returns
+ * once all have stopped); otherwise the host reads the flags after every pass.  passes_out
+ * (n ints, may be NULL) receives each case's passes; drop 0 runs every case max_pass passes. */
 smac_status smac_batch_run_until(smac_batch *b, int max_pass, double drop, int *passes_out);
 /* Device time (ms) of the forward mean-flow band sweep launch of the last pass of the last
  * smac_batch_run (the launch covering the most points: out[1] grid points, precision

@@ -1397,19 +1397,25 @@ static smac_status enqueue_batch_pass_multi(smac_batch *b, bool time_band)
 
 // one pass of every case, in the order of enqueue_pass for each case.  Case i's kernels of a
 // phase run on lane i % BATCH_LANES; the band sweeps of all cases on the batch stream.
-static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
+static smac_status enqueue_batch_pass(smac_batch *b, bool time_band,
+                                      const std::vector<char> *on = nullptr)
 {
     if (b->multi_ok) {
         bool later = true;
         for (smac_handle *h : b->cases) later = later && h->nt >= 1 && !(has_sa(h) && h->first_sa);
-        if (later) return enqueue_batch_pass_multi(b, time_band);
+        if (later && !on) return enqueue_batch_pass_multi(b, time_band);
     }
     hipStream_t sm = b->stream;
     auto lane_of = [&](int i) { return b->lane[i % BATCH_LANES]; };
+    // cases that stopped (smac_batch_run_until with a host-side test) get no kernels; their
+    // bands leave the multi-case sweeps at once on the device flags, which mirror `on`
+    auto runs = [&](int i) { return !on || (*on)[i] != 0; };
+    const int *act = on ? b->active : nullptr;
     const int nc = (int)b->cases.size();
     // SA up to its band sweeps (line-relaxation cases: the whole SA step)
     B_TRY(fork_lanes(b));
     for (int i = 0; i < nc; ++i) {
+        if (!runs(i)) continue;
         smac_handle *h = b->cases[i];
         hipStream_t ls = lane_of(i);
         h->nt += 1;
@@ -1431,18 +1437,20 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
     B_TRY(join_lanes(b));
     BandGroup &gs = b->sa;
     if (gs.nblk) {
-        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true, nullptr);
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, true, act);
         B_TRY(fork_lanes(b));
         for (size_t n = 0; n < gs.members.size(); ++n) {
+            if (!runs(gs.members[n])) continue;
             smac_handle *h = b->cases[gs.members[n]];
             lusgs_mid<double, 1>(lane_of(gs.members[n]), h->d, h->lws_sa);
         }
         B_TRY(join_lanes(b));
-        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false, nullptr);
+        lusgs_band_multi<double, 1>(sm, gs.jobs, gs.map, gs.nblk, gs.pf, false, act);
     }
     // SA after its sweeps; mean flow up to its band sweeps (line relaxation: the whole solve)
     B_TRY(fork_lanes(b));
     for (int i = 0; i < nc; ++i) {
+        if (!runs(i)) continue;
         smac_handle *h = b->cases[i];
         hipStream_t ls = lane_of(i);
         const Dims &d = h->d;
@@ -1482,12 +1490,13 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
         const bool rec = time_band && g == b->band_group;
         if (rec) HIP_TRY(hipEventRecord(b->ev_band[0], sm));
         if (g == 0)
-            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, nullptr);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, act);
         else
-            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, nullptr);
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, true, act);
         if (rec) HIP_TRY(hipEventRecord(b->ev_band[1], sm));
         B_TRY(fork_lanes(b));
         for (int i : gm.members) {
+            if (!runs(i)) continue;
             if (g == 0)
                 lusgs_mid<double, 3>(lane_of(i), b->cases[i]->d, b->cases[i]->lws);
             else
@@ -1495,12 +1504,13 @@ static smac_status enqueue_batch_pass(smac_batch *b, bool time_band)
         }
         B_TRY(join_lanes(b));
         if (g == 0)
-            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, nullptr);
+            lusgs_band_multi<double, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, act);
         else
-            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, nullptr);
+            lusgs_band_multi<float, 3>(sm, gm.jobs, gm.map, gm.nblk, gm.pf, false, act);
     }
     B_TRY(fork_lanes(b));
     for (int i = 0; i < nc; ++i) {
+        if (!runs(i)) continue;
         smac_handle *h = b->cases[i];
         hipStream_t ls = lane_of(i);
         const Dims &d = h->d;
@@ -1617,10 +1627,12 @@ smac_status smac_batch_run(smac_batch *b, int npass)
     return SMAC_OK;
 }
 
-// Runs up to max_pass passes; each case stops on the device after the pass whose residual
-// norm is at most drop times its first pass's (smac_run's criterion), so the host enqueues
-// passes without reading any case's statistics.  It looks at the cases' flags every 16
-// passes and returns once every case has stopped.
+// Runs up to max_pass passes; each case stops after the pass whose residual norm is at most
+// drop times its first pass's (smac_run's criterion).  With every case on LU-SGS the test is
+// on the device: the host enqueues passes without reading any case's statistics, looks at
+// the cases' flags every 16 passes and returns once every case has stopped.  Otherwise (the
+// per-case form of the pass, line-relaxation cases) the host reads the flags after every
+// pass and gives stopped cases no further kernels.
 smac_status smac_batch_run_until(smac_batch *b, int max_pass, double drop, int *passes_out)
 {
     if (!b) return fail(SMAC_ERR_ARG, "batch is NULL");
@@ -1631,8 +1643,6 @@ smac_status smac_batch_run_until(smac_batch *b, int max_pass, double drop, int *
         HIP_TRY(hipStreamSynchronize(h->stream_sa));
     }
     B_TRY(check_tables(b));
-    if (!b->multi_ok)
-        return fail(SMAC_ERR_ARG, "the device-side stop needs every case of the batch on LU-SGS");
     const int n = (int)b->cases.size();
     std::vector<int> flag(n, 1), cnt(n, 0), nt0(n);
     std::vector<double> first(n);
@@ -1643,11 +1653,20 @@ smac_status smac_batch_run_until(smac_batch *b, int max_pass, double drop, int *
     HIP_TRY(hipMemcpy(b->active, flag.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->passes, cnt.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->first, first.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<char> on(n, 1);
     for (int it = 0; it < max_pass; ++it) {
-        B_TRY(enqueue_batch_pass(b, false));
+        B_TRY(enqueue_batch_pass(b, false, b->multi_ok ? nullptr : &on));
         launch_batch_gate(b->stream, n, b->stp, b->first, drop, b->active, b->passes);
         HIP_TRY(hipGetLastError());
-        if ((it + 1) % 16 == 0 && it + 1 < max_pass) {
+        if (!b->multi_ok) {
+            HIP_TRY(hipMemcpy(flag.data(), b->active, n * sizeof(int), hipMemcpyDeviceToHost));
+            bool any = false;
+            for (int i = 0; i < n; ++i) {
+                on[i] = flag[i] != 0;
+                any = any || on[i];
+            }
+            if (!any) break;
+        } else if ((it + 1) % 16 == 0 && it + 1 < max_pass) {
             HIP_TRY(hipMemcpy(flag.data(), b->active, n * sizeof(int), hipMemcpyDeviceToHost));
             bool any = false;
             for (int v : flag) any = any || v != 0;

@@ -207,16 +207,31 @@ def test_device_side_stop_matches_smac_run():
             s.close()
 
 
-def test_device_side_stop_needs_lusgs_cases():
-    """The device-side stop works on the multi-case passes, which need every case on LU-SGS: a
-    batch with a line-relaxation case is refused, and its cases are left untouched."""
-    cases = [config_case(0, impsch=2), config_case(0, impsch=1)]
+def test_stop_with_line_relaxation_cases_matches_smac_run():
+    """smac_batch_run_until on a batch whose passes take the per-case form (line-relaxation
+    cases beside LU-SGS ones): the host reads the stop flags after every pass and gives a
+    stopped case no further kernels (its LU-SGS bands leave the multi-case sweeps on the
+    device flags); every case's pass count and state equal smac_run on it alone."""
+    from minismac_amd._abi import SMAC_ORDER_ZEBRA
+    steady = dict(reynum=1000.0, dtau=10.0, beta=10.0, alpha=4.0)
+    cases = [naca_case(119, 31, impsch=1, line_order=SMAC_ORDER_ZEBRA, underr=1.0, **steady),
+             naca_case(119, 31, impsch=1, line_order=SMAC_ORDER_ZEBRA, underr=1.0,
+                       precision=SMAC_FP32_LHS, **steady),
+             naca_case(119, 31, impsch=2, **steady),
+             config_case(0, impsch=1)]
+    max_pass, drop = 150, 5e-4
+    print(f"host-side stop: {len(cases)} cases, max_pass={max_pass}, drop={drop}", flush=True)
+    alone = [Solver(c) for c in cases]
     sol = [Solver(c) for c in cases]
     try:
+        done = [s.run(max_pass, drop)[1] for s in alone]
         with Batch(sol) as b:
-            with pytest.raises(SmacError):
-                b.run_until(10, 1e-3)
-        assert all(s.stats()["nt"] == 0 for s in sol)
+            passes = b.run_until(max_pass, drop)
+        assert passes == done, (passes, done)
+        assert max(done) == max_pass and min(done) < max_pass, done
+        for s0, s1 in zip(alone, sol):
+            assert all(np.array_equal(a, c) for a, c in zip(s0.get_state(), s1.get_state()))
+            assert s0.stats()["rhsnorm"] == s1.stats()["rhsnorm"]
     finally:
-        for s in sol:
+        for s in alone + sol:
             s.close()
