@@ -174,8 +174,12 @@ smac_status smac_timings(smac_handle *h, double out[7]);
  * factorisations), "adj_restart" (4..400, default 100: GMRES restart length of
  * smac_adjoint), "adj_reorth" (0..1000, default 100: its Gram-Schmidt projects a vector a
  * second time when less than this many thousandths of its square norm survive the first
- * projection; 500 is the DGKS test, 0 never).  All but the first two are tuning knobs;
- * results do not depend on them beyond the solver tolerances.
+ * projection; 500 is the DGKS test, 0 never), "lr_refactor" (1..64, default 1: the
+ * scan and zebra line-relaxation forms factor their lines every this many passes and keep
+ * the factored lines and neighbour blocks in between, with each pass's own residual: a
+ * lagged implicit operator with the same fixed point, not the reference's iteration).  All
+ * but the first two and the last are tuning knobs; results do not depend on them beyond the
+ * solver tolerances.
  * No counterpart in the reference. */
 smac_status smac_set_option(smac_handle *h, const char *name, int value);
 

@@ -71,6 +71,8 @@ struct smac_handle {
     int lusgs_rows = 64;
     int lusgs_ring = 1;    // the 3x3 sweeps stream their operands through an LDS ring
     int lr_lines = 64;     // lines per workgroup of the line-relaxation factor kernels
+    int lr_refactor = 1;   // scan/zebra forms: factor the lines every this many passes
+    long lr_factored = -1; // the pass whose blocks the line operands hold (-1: none)
     // line relaxation with each line solved by prefix scans inside the line (1), or by the
     // reference's sequential substitution (0, bit-identical to the reference)
     int lr_solve = 0;
@@ -456,6 +458,12 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
         h->lr_solve = value;
         return SMAC_OK;
     }
+    if (!std::strcmp(name, "lr_refactor")) {
+        if (value < 1 || value > 64) return fail(SMAC_ERR_ARG, "lr_refactor must be in [1, 64]");
+        h->lr_refactor = value;
+        h->lr_factored = -1;
+        return SMAC_OK;
+    }
     if (!std::strcmp(name, "lr_lines")) {
         if (value < 1 || value > 64) return fail(SMAC_ERR_ARG, "lr_lines must be in [1, 64]");
         h->lr_lines = value;
@@ -495,6 +503,7 @@ smac_status smac_set_state(smac_handle *h, const double *q, const double *anut, 
     HIP_TRY(hipMemcpy(h->st, &st, sizeof st, hipMemcpyHostToDevice));
     h->nt = nt;
     h->rhs0 = 0.0;
+    h->lr_factored = -1;
     if (nt > 0) h->first_sa = false;
     return SMAC_OK;
 }
@@ -523,6 +532,12 @@ static void enqueue_lrelax(smac_handle *h, hipStream_t sm)
     double *btc = h->blk[0], *bjm = h->blk[1], *bjp = h->blk[2], *bkm = h->blk[3],
            *bkp = h->blk[4];
     const bool zebra = h->p.line_order == SMAC_ORDER_ZEBRA;
+    // lr_refactor > 1: the lines keep the operands (factored line blocks and the neighbour
+    // blocks) of an earlier pass for up to that many passes, with this pass's residual; the
+    // fixed point s = 0 is unchanged (a lagged implicit operator, not the reference's)
+    const bool factor = h->lr_refactor <= 1 || h->lr_factored < 0 ||
+                        h->nt - h->lr_factored >= h->lr_refactor;
+    if (factor) h->lr_factored = h->nt;
     if (!h->lr_solve)
         launch_lrelax_ref<double>(sm, d, h->dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
                                   h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lr_lines);
@@ -530,10 +545,11 @@ static void enqueue_lrelax(smac_handle *h, hipStream_t sm)
         launch_lrelax_scan<float>(sm, d, h->dp, h->p.njsp, h->p.nksp, h->p.nbcimp, zebra, h->wake,
                                   h->q, h->dq, h->s, (const float *)btc, (const float *)bjm,
                                   (const float *)bjp, (const float *)bkm, (const float *)bkp,
-                                  h->lrs);
+                                  h->lrs, factor);
     else
         launch_lrelax_scan<double>(sm, d, h->dp, h->p.njsp, h->p.nksp, h->p.nbcimp, zebra,
-                                   h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lrs);
+                                   h->wake, h->q, h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lrs,
+                                   factor);
 }
 
 // one pass of the time loop body: SA, assembly, BC rows, residual, solve, update + forces

@@ -98,6 +98,22 @@ def test_fp32lhs_and_zebra_line_relaxation_converge_to_the_fp64_state():
         assert zeb[k] == pytest.approx(ref[k], rel=1e-7)
 
 
+def test_lagged_line_operands_converge_to_the_same_state():
+    """Option lr_refactor = N factors the zebra lines every N passes and keeps the operands in
+    between: a lagged implicit operator with the same fixed point, so the converged CL/CD
+    equal the every-pass form's (1e-7, as zebra against the reference order); N = 1 is the
+    every-pass form itself, bit for bit."""
+    zeb, n1 = _converge(_steady(line_order=SMAC_ORDER_ZEBRA))
+    one, n_one = _converge(_steady(line_order=SMAC_ORDER_ZEBRA), options={"lr_refactor": 1})
+    assert n_one == n1 and one["cl"] == zeb["cl"] and one["rhsnorm"] == zeb["rhsnorm"]
+    for n in (2, 4):
+        lag, nl = _converge(_steady(line_order=SMAC_ORDER_ZEBRA), options={"lr_refactor": n})
+        print(f"lr_refactor {n}: {nl} passes (every pass: {n1}); cl {lag['cl']!r} {zeb['cl']!r}",
+              flush=True)
+        for k in ("cl", "cd"):
+            assert lag[k] == pytest.approx(zeb[k], rel=1e-7)
+
+
 def test_fp32lhs_and_zebra_need_the_scan_form():
     for kw in (dict(precision=SMAC_FP32_LHS), dict(line_order=SMAC_ORDER_ZEBRA)):
         with Solver(_steady(**kw)) as g:
