@@ -456,6 +456,7 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
         if (value && (h->d.jm > 2048 || h->d.km > 2048))
             return fail(SMAC_ERR_SIZE, "the scan form of the line solves takes lines of up to 2048 points");
         h->lr_solve = value;
+        h->lr_factored = -1;   // the scan workspace holds no operands of this case yet
         return SMAC_OK;
     }
     if (!std::strcmp(name, "lr_refactor")) {
@@ -535,9 +536,9 @@ static void enqueue_lrelax(smac_handle *h, hipStream_t sm)
     // lr_refactor > 1: the lines keep the operands (factored line blocks and the neighbour
     // blocks) of an earlier pass for up to that many passes, with this pass's residual; the
     // fixed point s = 0 is unchanged (a lagged implicit operator, not the reference's)
-    const bool factor = h->lr_refactor <= 1 || h->lr_factored < 0 ||
+    const bool factor = !h->lr_solve || h->lr_refactor <= 1 || h->lr_factored < 0 ||
                         h->nt - h->lr_factored >= h->lr_refactor;
-    if (factor) h->lr_factored = h->nt;
+    h->lr_factored = !h->lr_solve ? -1 : (factor ? h->nt : h->lr_factored);
     if (!h->lr_solve)
         launch_lrelax_ref<double>(sm, d, h->dp, h->p.njsp, h->p.nksp, h->p.nbcimp, h->wake, h->q,
                                   h->dq, h->s, btc, bjm, bjp, bkm, bkp, h->lr_lines);
