@@ -235,3 +235,27 @@ def test_stop_with_line_relaxation_cases_matches_smac_run():
     finally:
         for s in alone + sol:
             s.close()
+
+
+def test_batch_with_lagged_line_operands_matches_eager_runs():
+    """Option lr_refactor in a batch: each line-relaxation case keeps its own lagged operands
+    (factored every N of its own passes), so its batched passes equal its passes alone."""
+    from minismac_amd._abi import SMAC_ORDER_ZEBRA
+    steady = dict(reynum=1000.0, dtau=10.0, beta=10.0, alpha=4.0, underr=1.0, impsch=1,
+                  line_order=SMAC_ORDER_ZEBRA)
+    cases = [naca_case(119, 31, **steady), naca_case(239, 61, precision=SMAC_FP32_LHS, **steady)]
+    opts = [{"lr_refactor": 3}, {"lr_refactor": 4}]
+    alone = [Solver(c, options=o) for c, o in zip(cases, opts)]
+    sol = [Solver(c, options=o) for c, o in zip(cases, opts)]
+    try:
+        for s in alone:
+            s.run(4)
+            s.run(7)
+        with Batch(sol) as b:
+            b.run(4)
+            b.run(7)
+        for s0, s1 in zip(alone, sol):
+            assert all(np.array_equal(a, c) for a, c in zip(s0.get_state(), s1.get_state()))
+    finally:
+        for s in alone + sol:
+            s.close()
