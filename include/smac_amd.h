@@ -172,7 +172,9 @@ smac_status smac_timings(smac_handle *h, double out[7]);
  * the 3x3 sweeps stream their operands through an LDS ring fed by loader wavefronts),
  * "lr_lines" (1..64, default 64: lines per workgroup of the line-relaxation
  * factorisations), "adj_restart" (4..400, default 100: GMRES restart length of
- * smac_adjoint), "adj_reorth" (0..1000, default 100: its Gram-Schmidt projects a vector a
+ * smac_adjoint), "adj_prec_steps" (1..8, default 3: steps of the transposed update per
+ * application of its GMRES preconditioner, with the operator between them; more steps, fewer
+ * iterations), "adj_reorth" (0..1000, default 100: its Gram-Schmidt projects a vector a
  * second time when less than this many thousandths of its square norm survive the first
  * projection; 500 is the DGKS test, 0 never), "lr_refactor" (1..64, default 1: the
  * scan and zebra line-relaxation forms factor their lines every this many passes and keep

@@ -88,6 +88,7 @@ struct smac_handle {
     int nt = 0;
     double rhs0 = 0.0;            // rhsnorm of pass nt = 1 when this handle made it (else 0)
     int adj_restart = 100;        // GMRES restart length of smac_adjoint
+    int adj_prec_steps = 3;       // steps of the transposed update per preconditioner call
     int adj_reorth = 100;         // per mille: when its Gram-Schmidt projects a second time
     bool first_sa = true;
     bool keep_lhs = false;
@@ -473,6 +474,11 @@ smac_status smac_set_option(smac_handle *h, const char *name, int value)
     if (!std::strcmp(name, "adj_reorth")) {
         if (value < 0 || value > 1000) return fail(SMAC_ERR_ARG, "adj_reorth must be in [0, 1000]");
         h->adj_reorth = value;
+        return SMAC_OK;
+    }
+    if (!std::strcmp(name, "adj_prec_steps")) {
+        if (value < 1 || value > 8) return fail(SMAC_ERR_ARG, "adj_prec_steps must be in [1, 8]");
+        h->adj_prec_steps = value;
         return SMAC_OK;
     }
     if (!std::strcmp(name, "adj_restart")) {
@@ -865,7 +871,7 @@ static smac_status adj_prepare(smac_handle *h, int m)
         c.rtype = rtd;
         c.inv = invd;
         double **v3[] = {&c.lam[0], &c.lam[1], &c.g[0], &c.g[1], &c.fc, &c.fa, &c.w, &c.z,
-                         &c.t1, &c.t2, &c.t1b, &c.fb, &c.smu};
+                         &c.t1, &c.t2, &c.t1b, &c.fb, &c.smu, &c.pk1, &c.pk2};
         if ((rc = dalloc(h, &c.jac, 81 * N))) return rc;
         for (int b = 0; b < 5; ++b)
             if ((rc = dalloc(h, &c.tb[b], 9 * N))) return rc;
@@ -935,6 +941,9 @@ smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const dou
     smac_status rc = adj_prepare(h, m);
     if (rc) return rc;
     const Dims &d = h->d;
+    // CL, CD and the far field's circulation of this state (after a run, the same values its
+    // last pass left; after smac_set_state, none has been computed for it yet)
+    launch_force(h->stream, d, h->dp, h->walls, h->x, h->y, h->met, h->dj, h->q, h->w, h->st);
     // the FP64 residual and blocks of the state (the iteration's own operator M)
     launch_assemble<double>(h->stream, d, h->dp, h->q, h->met, h->dj, h->s, h->blk[0], h->blk[1],
                             h->blk[2], h->blk[3], h->blk[4]);
@@ -946,6 +955,7 @@ smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const dou
     HIP_TRY(hipStreamSynchronize(h->stream));
     AdjCtx &c = h->adj;
     c.reorth = h->adj_reorth * 1e-3;
+    c.psteps = h->adj_prec_steps;
     c.clift = ds.clift;
     c.cl = ds.cl;
     c.cd = ds.cd;

@@ -186,6 +186,9 @@ struct AdjCtx {
     int m;                       // GMRES restart length
     double reorth;               // second Gram-Schmidt pass when less than this fraction of
                                  // the vector's square norm survives the first (0: never)
+    int psteps;                  // preconditioner: this many steps of the transposed update
+                                 // (with the operator between them)
+    double *pk1, *pk2;           // its work vectors (3 planes each)
     const signed char *rtype;    // row type per point
     int npair;
     const int *pairs, *inv;      // wake pairs (0-based) and, per pair, the pair whose partner

@@ -203,6 +203,20 @@ def test_adjoint_gradients_match_the_cpu_tangent(steady_gradients):
     np.testing.assert_allclose(dcd, tg[1], rtol=0, atol=2e-6 * np.abs(tg[1]).max())
 
 
+def test_adjoint_preconditioner_steps_agree(steady_gradients):
+    """adj_prec_steps: one step of the transposed update per preconditioner application (the
+    plain form) and the default three give the same gradients to the solve tolerance, the
+    latter in fewer GMRES iterations."""
+    case, modes, st, q, dcl, dcd, ast = steady_gradients
+    with Solver(case, options={"adj_prec_steps": 1}) as g:
+        g.set_state(q=q, nt=5, resmax0=1.0, clift=st["cl"])
+        dcl1, dcd1, ast1 = g.adjoint(modes, maxit=8000, tol=1e-12)
+    print(f"iterations: 1 step {ast1['iters']}, 3 steps {ast['iters']}", flush=True)
+    assert all(a < b for a, b in zip(ast["iters"], ast1["iters"]))
+    np.testing.assert_allclose(dcl, dcl1, rtol=0, atol=1e-8 * np.abs(dcl1).max())
+    np.testing.assert_allclose(dcd, dcd1, rtol=0, atol=1e-8 * np.abs(dcd1).max())
+
+
 def test_adjoint_gradients_match_central_differences(steady_gradients):
     case, modes, st, q, dcl, dcd, ast = steady_gradients
     h = 2e-2
