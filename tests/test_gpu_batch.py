@@ -259,3 +259,26 @@ def test_batch_with_lagged_line_operands_matches_eager_runs():
     finally:
         for s in alone + sol:
             s.close()
+
+
+def test_device_side_stop_at_full_size():
+    """The device-side stop at the workload grid's size: 945x241 and 477x121 steady cases
+    (FP32-LHS and FP64) batched to a residual drop equal their smac_run alone."""
+    steady = dict(reynum=1000.0, dtau=10.0, beta=10.0, impsch=2)
+    cases = [naca_case(945, 241, m=0.04, alpha=8.0, wall_spacing=2e-6,
+                       precision=SMAC_FP32_LHS, **steady),
+             naca_case(477, 121, m=0.02, alpha=4.0, wall_spacing=5e-6, **steady)]
+    max_pass, drop = 80, 0.05
+    alone = [Solver(c) for c in cases]
+    sol = [Solver(c) for c in cases]
+    try:
+        done = [s.run(max_pass, drop)[1] for s in alone]
+        with Batch(sol) as b:
+            passes = b.run_until(max_pass, drop)
+        print(f"full-size stop: passes {passes}", flush=True)
+        assert passes == done, (passes, done)
+        for s0, s1 in zip(alone, sol):
+            assert all(np.array_equal(a, c) for a, c in zip(s0.get_state(), s1.get_state()))
+    finally:
+        for s in alone + sol:
+            s.close()
