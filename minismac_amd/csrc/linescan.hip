@@ -838,7 +838,7 @@ k_lr_tspread(int nline, int len, long N, int par, double w, double *__restrict__
     const LT *pOm = op + 27 * N, *pOp = op + 36 * N;
     for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < N;
          o += (long)gridDim.x * blockDim.x) {
-        const int l = (int)(o / len);
+        const int l = (int)o / len;   // o < N < 2^31: a 32-bit division
         if ((l & 1) == par) {
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
