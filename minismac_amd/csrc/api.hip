@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/smac_amd.h"
@@ -82,6 +83,8 @@ struct smac_handle {
     int adj_m = 0;
     AdjCtx adj = {};
     LusgsWs adj_ws = {};
+    LusgsWs adj_ws2 = {};      // the second adjoint solve's (LU-SGS preconditioner)
+    AdjCtx adj2 = {};          // the second adjoint solve's work vectors (the rest is adj's)
     std::vector<void *> lws_allocs;
     Walls walls = {};
     std::vector<BcRegion> regs;
@@ -139,6 +142,7 @@ static void free_lusgs_ws(smac_handle *h)
     h->lws = LusgsWs{};
     h->lws_sa = LusgsWs{};
     h->adj_ws = LusgsWs{};
+    h->adj_ws2 = LusgsWs{};
 }
 
 static smac_status alloc_one_ws(smac_handle *h, int rows, int neqs, LusgsWs &w)
@@ -878,6 +882,16 @@ static smac_status adj_prepare(smac_handle *h, int m)
         for (double **pp : v3)
             if ((rc = dalloc(h, pp, 3 * N))) return rc;
         if ((rc = dalloc(h, &c.V, 3 * N * (m + 1)))) return rc;
+        // the CD adjoint runs beside the CL adjoint with its own work vectors
+        AdjCtx &c2 = h->adj2;
+        c2 = AdjCtx{};
+        double **w3[] = {&c2.w, &c2.z, &c2.t1, &c2.t2, &c2.t1b, &c2.pk1, &c2.pk2};
+        for (double **pp : w3)
+            if ((rc = dalloc(h, pp, 3 * N))) return rc;
+        if ((rc = dalloc(h, &c2.V, 3 * N * (m + 1))) || (rc = dalloc(h, &c2.part, 512 * (m + 2))) ||
+            (rc = dalloc(h, &c2.red, m + 2)) || (rc = dalloc(h, &c2.red2, m + 2)) ||
+            (rc = dalloc(h, &c2.gate, 1)))
+            return rc;
         if ((rc = dalloc(h, &c.dmet, 4 * N)) || (rc = dalloc(h, &c.ddj, N)) ||
             (rc = dalloc(h, &c.dxy, 2 * N)) || (rc = dalloc(h, &c.fpart, 2 * h->d.jm)) ||
             (rc = dalloc(h, &c.part, 512 * (m + 2))) || (rc = dalloc(h, &c.red, m + 2)) ||
@@ -897,17 +911,22 @@ static smac_status adj_prepare(smac_handle *h, int m)
             if ((rc = dalloc(h, &sv, 6 * N))) return rc;
             c.lr_f.s_t = c.lr_t.s_t = sv;
             c.lr_f.dq_t = c.lr_t.dq_t = sv + 3 * N;
-            double **lv[] = {&c.lrv.a, &c.lrv.z, &c.lrv.at, &c.lrv.st, &c.lrv.zt};
+            double **lv[] = {&c.lrv.a, &c.lrv.z, &c.lrv.at, &c.lrv.st, &c.lrv.zt,
+                             &c2.lrv.a, &c2.lrv.z, &c2.lrv.at, &c2.lrv.st, &c2.lrv.zt};
             for (double **pp : lv)
                 if ((rc = dalloc(h, pp, 3 * N))) return rc;
-        } else if (!h->adj_ws.lhs) {
-            h->adj_ws.ring = 1;
-            rc = alloc_one_ws(h, 64, 3, h->adj_ws);
-            if (rc) return rc;
         }
         h->adj_m = m;
         h->adj_ready = true;
     }
+    // the LU-SGS preconditioner's workspaces (one per solve; a change of lusgs_rows frees them)
+    if (h->p.impsch != 1)
+        for (LusgsWs *ws : {&h->adj_ws, &h->adj_ws2})
+            if (!ws->lhs) {
+                ws->ring = 1;
+                smac_status rc = alloc_one_ws(h, 64, 3, *ws);
+                if (rc) return rc;
+            }
     AdjCtx &c = h->adj;
     c.d = h->d;
     c.sm = h->stream;
@@ -962,7 +981,24 @@ smac_status smac_adjoint(smac_handle *h, int nshape, const double *dx, const dou
     adj_build(c);
     double rel[2] = {0.0, 0.0};
     int its[2] = {0, 0};
-    for (int o = 0; o < 2; ++o) its[o] = adj_gmres(c, c.g[o], c.lam[o], maxit, tol, &rel[o]);
+    // The two solves run side by side, CL's on the handle's stream and CD's on its second
+    // stream from a host thread of its own: each GMRES iteration waits for its Hessenberg
+    // column on the host, and the other solve's kernels fill that gap.  They share the
+    // operator and the preconditioner's factors (read only) and have their own work vectors.
+    AdjCtx c2 = c;
+    {
+        const AdjCtx &b2 = h->adj2;
+        c2.sm = h->stream_sa;
+        c2.V = b2.V; c2.w = b2.w; c2.z = b2.z; c2.t1 = b2.t1; c2.t2 = b2.t2; c2.t1b = b2.t1b;
+        c2.pk1 = b2.pk1; c2.pk2 = b2.pk2; c2.part = b2.part; c2.red = b2.red;
+        c2.red2 = b2.red2; c2.gate = b2.gate; c2.lrv = b2.lrv;
+        c2.ws = h->adj_ws2;
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));   // the operator and factors are built
+    std::thread cd([&]() { its[1] = adj_gmres(c2, c.g[1], c.lam[1], maxit, tol, &rel[1]); });
+    its[0] = adj_gmres(c, c.g[0], c.lam[0], maxit, tol, &rel[0]);
+    cd.join();
+    HIP_TRY(hipStreamSynchronize(h->stream_sa));
     adj_gradients(c, nshape, dx, dy, dcl, dcd);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
