@@ -1716,9 +1716,27 @@ smac_status smac_batch_run_until(smac_batch *b, int max_pass, double drop, int *
     HIP_TRY(hipMemcpy(b->active, flag.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->passes, cnt.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->first, first.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    // each case's pass count and reference norm from the device (also after a failed pass,
+    // so that the handles count what ran)
+    auto settle = [&]() -> hipError_t {
+        hipError_t e = hipStreamSynchronize(b->stream);
+        if (e == hipSuccess) e = hipMemcpy(cnt.data(), b->passes, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(first.data(), b->first, n * sizeof(double), hipMemcpyDeviceToHost);
+        for (int i = 0; i < n; ++i) {
+            smac_handle *h = b->cases[i];
+            h->nt = nt0[i] + cnt[i];   // the host counted every enqueued pass
+            if (nt0[i] == 0 && cnt[i] > 0) h->rhs0 = first[i];
+            if (passes_out) passes_out[i] = cnt[i];
+        }
+        return e;
+    };
     std::vector<char> on(n, 1);
     for (int it = 0; it < max_pass; ++it) {
-        B_TRY(enqueue_batch_pass(b, false, b->multi_ok ? nullptr : &on));
+        smac_status rc = enqueue_batch_pass(b, false, b->multi_ok ? nullptr : &on);
+        if (rc) {
+            (void)settle();
+            return rc;
+        }
         launch_batch_gate(b->stream, n, b->stp, b->first, drop, b->active, b->passes);
         HIP_TRY(hipGetLastError());
         if (!b->multi_ok) {
@@ -1736,15 +1754,7 @@ smac_status smac_batch_run_until(smac_batch *b, int max_pass, double drop, int *
             if (!any) break;
         }
     }
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    HIP_TRY(hipMemcpy(cnt.data(), b->passes, n * sizeof(int), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(first.data(), b->first, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) {
-        smac_handle *h = b->cases[i];
-        h->nt = nt0[i] + cnt[i];   // the host counted every enqueued pass
-        if (nt0[i] == 0 && cnt[i] > 0) h->rhs0 = first[i];
-        if (passes_out) passes_out[i] = cnt[i];
-    }
+    HIP_TRY(settle());
     b->band_ms = 0.0;
     b->band_pts = 0.0;
     b->band_group = -1;
