@@ -102,6 +102,25 @@ def test_adjoint_jacobian_matches_oracle_differences(ivis, tol):
         assert np.abs(jt_w[i].ravel(order="F") - want).max() <= 1e-13 * np.abs(want).max()
 
 
+def test_lusgs_adjoint_solves_side_by_side_agree_with_one_step():
+    """The LU-SGS-preconditioned adjoint (each of the two side-by-side solves with its own
+    band workspace): the CL and CD solves at a 119x31 LU-SGS state give the same dCL/dalpha
+    and dCD/dalpha with one preconditioner step and with three, to the solve tolerance."""
+    case = config_case(0, impsch=2, reynum=1000.0, dtau=10.0)
+    out = []
+    for k in (1, 3):
+        with Solver(case, options={"adj_prec_steps": k}) as g:
+            g.run(40)
+            dcl, dcd, ast = g.adjoint((), maxit=6000, tol=1e-10)
+        print(f"LU-SGS adjoint, {k} step(s): iterations {ast['iters']}, resid {ast['resid']}",
+              flush=True)
+        assert max(ast["resid"]) < 1e-9
+        out.append((dcl[0], dcd[0], ast["iters"]))
+    (cl1, cd1, it1), (cl3, cd3, it3) = out
+    assert cl3 == pytest.approx(cl1, rel=1e-7) and cd3 == pytest.approx(cd1, rel=1e-7)
+    assert all(a < b for a, b in zip(it3, it1))
+
+
 def test_adjoint_update_is_the_primal_update_and_its_transpose():
     case, o, ost = _unsteady_state(1)
     q = o.array("q")
