@@ -181,7 +181,15 @@ k_lr_factor(Dims d, int nx, int twist, const LT *__restrict__ btc, const LT *__r
     const bool eta = (int)blockIdx.x >= nx;
     // the xi lines are the longer chains: their waves win issue slots on a shared SIMD
     if (!eta) __builtin_amdgcn_s_setprio(2);
-    const int l = eta ? (int)blockIdx.x - nx : (int)blockIdx.x;
+    // eta lines: a point's nine planes are read by lanes jm apart, and the lines of
+    // neighbouring j share those cache lines; workgroups reach the 8 XCDs round-robin, so
+    // each XCD takes a contiguous run of j (its own L2 then holds the shared lines)
+    int l = (int)blockIdx.x;
+    if (eta) {
+        const int t = (int)blockIdx.x - nx, ne = (int)gridDim.x - nx;
+        const int r = t % 8, q = ne / 8, rem = ne % 8;
+        l = r * q + (r < rem ? r : rem) + t / 8;
+    }
     const int len = eta ? d.km : d.jm;
     // twist = 0: the plain (one-sided) factorisation, forward over the whole line (wave 0;
     // launched with one wave)
