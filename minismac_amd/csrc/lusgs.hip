@@ -483,15 +483,6 @@ SMAC_DEV void edge_st(double v, rsrc_t r, unsigned vo, unsigned so)
                                           EDGE_POLICY);
 }
 
-// two adjacent slot words in one access (each word is still checked on its own)
-struct U2 {
-    unsigned long long x, y;
-};
-SMAC_DEV U2 edge_ld2(rsrc_t r, unsigned vo, unsigned so)
-{
-    return __builtin_bit_cast(U2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so,
-                                                                        EDGE_POLICY));
-}
 
 
 // Re-reads one hand-over slot and waits for it.  Written as one asm block so that the
@@ -771,9 +762,11 @@ struct Band {
         // padded: rows outside 0..jm-1 read finite padding, bands without a neighbour read
         // a region of zeros
         if constexpr (NEQ == 3) {
-            const U2 e01 = edge_ld2(rEin, 0, so);
-            sl.e[0] = e01.x;
-            sl.e[1] = e01.y;
+            // one 8-byte load per word: a wider load is not guaranteed to see each word
+            // whole, and a word read half written (the sentinel's high half, a NaN, with the
+            // value's low half) would pass the sentinel test
+            sl.e[0] = edge_ld(rEin, 0, so);
+            sl.e[1] = edge_ld(rEin, 8, so);
             sl.e[2] = edge_ld(rEin, 16, so);
         } else {
             sl.e[0] = edge_ld(rEin, 0, so);
