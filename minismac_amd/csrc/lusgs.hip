@@ -36,6 +36,16 @@
 namespace smac {
 
 constexpr unsigned long long EDGE_EMPTY = 0x7ff4dead0000beefull;   // signalling NaN
+
+// A slot word counts as not yet written when its high half is the sentinel's: a hand-over
+// value is finite, so its high half never is (0x7ff4dead is a NaN exponent), and a word read
+// while it was being written, with the sentinel's high half and the value's low half, is
+// then waited for and read again instead of taken as a NaN.
+SMAC_DEV bool edge_empty(unsigned long long v)
+{
+    return (unsigned)(v >> 32) == (unsigned)(EDGE_EMPTY >> 32);
+}
+
 constexpr int EDGE_SPIN_MAX = 200000;
 constexpr int BAND_LANES = 64;
 constexpr int HP0 = 16;          // empty hyperplanes before h = 0 and after h = H
@@ -504,7 +514,7 @@ SMAC_DEV unsigned long long edge_reload(const unsigned long long *p)
 SMAC_DEV void edge_lag(const unsigned long long *p, bool &broken, int *stall)
 {
     int n = 0;
-    while (edge_reload(p) == EDGE_EMPTY) {
+    while (edge_empty(edge_reload(p))) {
         if (++n > EDGE_SPIN_MAX) {
             broken = true;
             atomicOr(stall, 1);
@@ -525,7 +535,7 @@ SMAC_DEV void edge_wait(const unsigned long long *p, unsigned long long e[NEQ], 
     for (;;) {
         bool ok = true;
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) ok = ok && e[m] != EDGE_EMPTY;
+        for (int m = 0; m < NEQ; ++m) ok = ok && !edge_empty(e[m]);
         if (ok) return;
         if (++n > EDGE_SPIN_MAX) {
             broken = true;
@@ -794,7 +804,7 @@ struct Band {
         T nb[NEQ];
         bool empty = false;
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) empty = empty || c.e[m] == EDGE_EMPTY;
+        for (int m = 0; m < NEQ; ++m) empty = empty || edge_empty(c.e[m]);
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(empty) != 0, 0))   // uniform; rare
             edge_wait<NEQ>(ein_ptr + (long)(h - kin + EPAD) * 4, c.e, broken, stall);
         if constexpr (RING && !SEL) {
