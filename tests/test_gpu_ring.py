@@ -81,7 +81,9 @@ def test_fp32_sweeps_repeat_bit_identical(ring):
     16-byte store of two hand-over words with an SGPR offset, whose data registers the next
     instruction overwrote, used to put the third word in the first now and then (about one
     run in a hundred of three passes at 945x241, both sweep forms); the slots are now written
-    one 8-byte word per store."""
+    one 8-byte word per store.  A run that goes non-finite fails with the pass and the
+    places of the non-finite values: a slot word read half written could carry the sentinel's
+    NaN high half, which the sweeps now treat as not yet written."""
     case = naca_case(945, 241, m=0.04, alpha=8.0, reynum=3e6, beta=10.0, impsch=2,
                      wall_spacing=2e-6, precision=SMAC_FP32_LHS)
     print(f"repeated FP32-LHS sweeps, lusgs_ring={ring}", flush=True)
